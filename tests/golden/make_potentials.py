@@ -1,0 +1,152 @@
+"""Write the synthetic ænet potentials the workloads use (tests/golden/potentials/).
+
+Run from the repository root:  python tests/golden/make_potentials.py
+
+Weights are drawn N(0, 1/fan_in) with fixed seeds (fan_in = number of nodes
+of the previous layer); normalization statistics (sfval_avg = mean,
+sfval_cov = mean of squares, sfval_min/max) are the FP64 statistics of the
+unscaled descriptors over seeded calibration cells, computed by the oracle
+(oracle/_build/liboracle.so).  The files are written in the reference's
+.nn format so that the reference library loads them unchanged.
+"""
+import os
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+from aenet_amd import nnfile, synth  # noqa: E402
+from oracle_lib import Oracle  # noqa: E402
+
+OUT = os.path.join(ROOT, "tests", "golden", "potentials")
+
+
+def random_network(arch, seed, acts=None):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    blocks = []
+    for l in range(len(arch) - 1):
+        n1, n2 = arch[l], arch[l + 1]
+        # column-major (n2, n1+1) block, last column = bias
+        blocks.append(rng.normal(0.0, 1.0 / np.sqrt(n1), size=(n1 + 1) * n2))
+    if acts is None:
+        acts = [1] * (len(arch) - 2) + [0]
+    return np.concatenate(blocks), acts
+
+
+def make_pot(atomtype, setup, arch, seed, types, e_atoms, acts=None):
+    W, fa = random_network(arch, seed, acts)
+    pot = dict(setup)
+    pot.update({
+        "nnodes": arch, "f_a": fa, "W": W, "description": "synthetic potential (seed %d)" % seed,
+        "sfval_min": np.zeros(setup["nsf"]), "sfval_max": np.zeros(setup["nsf"]),
+        "sfval_avg": np.zeros(setup["nsf"]), "sfval_cov": np.ones(setup["nsf"]),
+        "ts_file": "synthetic", "ts_normalized": True, "ts_scale": 1.0, "ts_shift": 0.0,
+        "ts_types": list(types), "ts_E_atom": list(e_atoms), "ts_natoms": 0, "ts_nstrucs": 0,
+        "ts_E": [0.0, 0.0, 0.0],
+    })
+    pot["atomtype"] = atomtype
+    return pot
+
+
+def calibrate(pots, types, structures):
+    """Fill sfval_* of every potential from unscaled descriptors of the
+    atoms of its species in the calibration structures."""
+    with tempfile.TemporaryDirectory() as d:
+        files = {}
+        for t in types:
+            files[t] = os.path.join(d, t + ".nn")
+            nnfile.write_nn(files[t], pots[t])
+        orc = Oracle(types, files)
+        vals = {t: [] for t in types}
+        for s in structures:
+            nb = orc.neighbors(s.cell, s.pbc, s.types, s.positions)
+            L = nb["latt"].T  # columns = lattice vectors
+            for i in range(len(s)):
+                coo_i = L @ nb["frac"][i]
+                # recompute the centre exactly as the neighbor list frames it
+                n = nb["nnb"][i]
+                G = orc.descriptors(int(s.types[i]), coo_i, nb["coo"][i, :n], nb["type"][i, :n],
+                                    scaled=False)
+                vals[types[s.types[i] - 1]].append(G)
+    for t in types:
+        G = np.array(vals[t])
+        p = pots[t]
+        p["sfval_min"] = G.min(axis=0)
+        p["sfval_max"] = G.max(axis=0)
+        p["sfval_avg"] = G.mean(axis=0)
+        p["sfval_cov"] = (G * G).mean(axis=0)
+        p["neval"] = len(G)
+
+
+def write_set(name, types, pots):
+    paths = {}
+    for t in types:
+        path = os.path.join(OUT, "%s.%s.nn" % (name, t))
+        nnfile.write_nn(path, pots[t])
+        paths[t] = path
+    return paths
+
+
+def cheb_set(name, types, e_atoms, rr, rn, ar, an, hidden, seed0, calib):
+    pots = {}
+    for k, t in enumerate(types):
+        st = nnfile.chebyshev_setup(t, types, rr, rn, ar, an)
+        arch = [st["nsf"]] + list(hidden) + [1]
+        pots[t] = make_pot(t, st, arch, seed0 + k, types, e_atoms)
+    calibrate(pots, types, calib)
+    return write_set(name, types, pots)
+
+
+def bp_functions(types):
+    fns = []
+    etas2 = np.geomspace(0.003, 0.5, 8)
+    for t2 in types:
+        for eta in etas2:
+            fns.append({"G": 2, "type2": t2, "Rc": 6.0, "Rs": 0.0, "eta": float(eta)})
+    for a in range(len(types)):
+        for b in range(a, len(types)):
+            for zeta in (1.0, 2.0, 4.0):
+                for lam in (-1.0, 1.0):
+                    for eta in (0.0003, 0.03, 0.09):
+                        fns.append({"G": 4, "type2": types[a], "type3": types[b], "Rc": 6.0,
+                                    "lambda": lam, "zeta": zeta, "eta": eta})
+    return fns
+
+
+def main():
+    os.makedirs(OUT, exist_ok=True)
+    # fcc108: single species, Chebyshev radial 16 / 6.0 A, angular 4 / 4.0 A, 22-20-20-1
+    c, p = synth.fcc(3.61, (3, 3, 3))
+    cal = [synth.Structure(c, synth.displaced(p, 0.1, 3), np.ones(len(p), np.int32))]
+    cheb_set("fcc108", ["Cu"], [-1000.0], 6.0, 16, 4.0, 4, (20, 20), 2, cal)
+    # alloy: A/B with typespin, radial 18 / 6.5 A, angular 6 / 4.5 A, 52-30-30-1
+    cal = [synth.binary_fcc(3.8, (5, 5, 5), 0.08, 13, 14)]
+    cheb_set("alloy", ["A", "B"], [-1000.0, -1500.0], 6.5, 18, 4.5, 6, (30, 30), 4, cal)
+    # teacher for Monte Carlo: fcc108 descriptor shapes for A/B (44 features), 44-20-20-1
+    cal = [synth.binary_fcc(3.61, (4, 4, 4), 0.1, 15, 16)]
+    cheb_set("teacher", ["A", "B"], [-1000.0, -1500.0], 6.0, 16, 4.0, 4, (20, 20), 6, cal)
+    # student: radial-only Chebyshev order 8 / 5.0 A (angular order 0, Rc 0), 20-8-1
+    cheb_set("student", ["A", "B"], [-1000.0, -1500.0], 5.0, 8, 0.0, 0, (8,), 30, cal)
+    # Behler-Parrinello, 3 species: 8 G2 per species + 18 G4 per species pair = 132
+    types = ["A", "B", "C"]
+    pots = {}
+    fns = bp_functions(types)
+    for k, t in enumerate(types):
+        st = nnfile.behler_setup(t, types, fns)
+        pots[t] = make_pot(t, st, [st["nsf"], 25, 25, 1], 8 + k, types, [-1000.0, -1500.0, -800.0])
+    cal = []
+    for seed in (24, 25):
+        c, p = synth.fcc(3.6, (3, 3, 3))
+        rng = np.random.Generator(np.random.PCG64(seed))
+        cal.append(synth.Structure(c, synth.displaced(p, 0.1, seed + 10), rng.integers(1, 4, len(p))))
+    calibrate(pots, types, cal)
+    write_set("bp3", types, pots)
+    print("wrote potentials to", OUT)
+
+
+if __name__ == "__main__":
+    main()
