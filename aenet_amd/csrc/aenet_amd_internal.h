@@ -1,0 +1,167 @@
+// aenet_amd_internal.h -- data layouts shared by the host code and the
+// gfx950 kernels.  Plain structs only; no torch types.
+#pragma once
+
+#include <stdint.h>
+
+#define AENET_AMD_MAX_TYPES 8
+#define AENET_AMD_MAX_LAYERS 8
+#define AENET_AMD_MAX_HIDDEN 64      // hidden layer width handled by one wavefront
+#define AENET_AMD_MAX_NSF 256        // descriptor length per species
+#define AENET_AMD_NB_CAP 512         // neighbors held in LDS per atom
+#define AENET_AMD_KEPT_CAP 512       // (cell, image) combos kept per atom
+#define AENET_AMD_COMBO_CAP 2048     // (cell, image) combos per home cell
+
+// error flags raised by kernels (bit mask in a device int)
+#define AENET_AMD_FLAG_NB_OVERFLOW 1
+#define AENET_AMD_FLAG_COMBO_OVERFLOW 2
+#define AENET_AMD_FLAG_KEPT_OVERFLOW 4
+#define AENET_AMD_FLAG_BAD_TYPE 8
+#define AENET_AMD_FLAG_TRIPLET_OVERFLOW 16
+#define AENET_AMD_FLAG_BAD_COORD 32
+
+enum { DESC_CHEBYSHEV = 0, DESC_BEHLER = 1 };
+
+// Per-species parameters (one per global atom type).  Offsets index the
+// model's float weight array and double normalization arrays.
+struct SpeciesDev {
+  int nsf;
+  int nlayers;
+  int nnodes[AENET_AMD_MAX_LAYERS];
+  int fa[AENET_AMD_MAX_LAYERS];    // activation of layer l (l >= 1)
+  int w_off[AENET_AMD_MAX_LAYERS]; // start of the weight block of layer l -> l+1
+  int norm_off;
+  // Chebyshev basis (new_SFBasis layout; 0-based feature offsets)
+  int r_N, a_N, multi;
+  int r_i1, a_i1, r_i2, a_i2;
+  float r_Rc, a_Rc;
+  float spin[AENET_AMD_MAX_TYPES]; // typespin of neighbor global type t (index t-1)
+  // Behler-Parrinello: row of the symmetry-function tables used for this
+  // central species and the local type of each global neighbor type
+  int bp_row;
+  int bp_ltype[AENET_AMD_MAX_TYPES];
+  double E_scale, E_shift, E_atom;
+};
+
+// Behler-Parrinello tables (sf_init layout, 0-based): per (row, local type2)
+// a radial block, per (row, local type2 <= type3) an angular block.
+struct BPDev {
+  int ntypes;
+  int rad_off[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];   // feature offset of radial block
+  int rad_n[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];     // number of G2 (G1/G3 folded in kind)
+  int rad_p0[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];    // first parameter record
+  int ang_off[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];
+  int ang_n[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];
+  int ang_p0[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];
+};
+
+// one radial (G1/G2/G3) or angular (G4/G5) function record
+struct BPFunc {
+  int kind;      // 1..5
+  float Rc, p1, p2, p3;   // G2: Rs, eta; G3: kappa; G4/G5: lambda, zeta, eta
+};
+
+// Per-structure plan: lattice-derived data computed on the host with the
+// reference's arithmetic, plus offsets into the batch's flat arrays.
+struct StructDev {
+  double L[9];       // lattice, column k = vector k
+  double B[9];       // 2*pi-scaled reciprocal lattice (geo_recip_lattice)
+  double shift[3];   // isolated structures: bounding-box origin
+  double frac_div;   // 2*pi (Cartesian input) or 1 (lattice coordinates given)
+  double Rc2;        // (Rc_max + 1e-3)^2
+  double rc_list;    // Rc_max + 1e-3
+  double cell_rad;   // circumradius of one grid cell (all cells of a structure alike)
+  int pbc;
+  int atom0, natoms;
+  int nc[3];
+  int cell0, ncells;
+  int nT, T0;        // lattice translations (3 ints each) at T0
+  int nC, C0;        // cell stencil vectors at C0
+  int nself, self0;  // self images: Tvec ranks (1-based) and distances at self0
+};
+
+// sizes an atom kernel launch depends on
+struct AtomSizes {
+  int natoms;   // central atoms (structures) or environments
+  int cap;      // neighbor capacity per atom
+  int nsf_max;  // longest descriptor
+  int wsize;    // total network weights
+  int nl_max;   // most network layers (input layer included)
+};
+
+struct ModelDev {
+  const SpeciesDev *species;  // [ntypes]
+  const float *W;             // network weights
+  const double *norm_avg;     // [sum nsf]
+  const double *norm_scale;
+  const BPDev *bp;
+  const BPFunc *bpf;
+  int ntypes;
+  int desc;
+};
+
+struct BatchDev {
+  const StructDev *st;
+  int nstruct;
+  int natoms;
+  const int *atom_struct;     // [natoms]
+  const double *cart;         // [3*natoms]
+  const int *types;           // [natoms], 1-based
+  double *s;                  // [3*natoms] wrapped lattice coordinates
+  int *cellof;                // [natoms] global cell id
+  int ncells;                 // total cells
+  const int *cell_struct;     // [ncells]
+  const int *cell_ijk;        // [3*ncells] 1-based cell indices
+  const double *cell_ctr;     // [3*ncells] Cartesian cell centres (fixed per plan)
+  int *cell_count;            // [ncells]
+  int *cell_fill;             // [ncells]
+  int *cell_start;            // [ncells+1]
+  int *cell_atoms;            // [natoms] atom ids, cells in order, descending id within a cell
+  double *sorted_s;           // [3*natoms] lattice coordinates in cell_atoms order
+  const int *combo_count;     // [ncells] (cell, image) combinations of each home cell
+  const int *combos;          // [2*ncells*combo_cap] pairs (cell, image rank), visiting order
+  int combo_cap;
+  const int *Tvec;            // all lattice translations
+  const int *Cvec;            // all stencil vectors
+  const int *self_t;          // self-image ranks
+  const double *self_d;       // self-image distances
+  int *flags;
+};
+
+// compact neighbor lists written by the search kernel for the atom kernels
+struct NbDev {
+  int cap;
+  int *count;          // [natoms]
+  float *R;            // [4*natoms*cap]: R_ij (A) and |R_ij|
+  int *j;              // [natoms*cap] global atom index of the neighbor
+};
+
+// outputs of the atom kernel
+struct OutDev {
+  double *E_atom;      // [natoms] atomic energies (incl. E_atom offset)
+  double *F;           // [3*natoms] forces, accumulated with FP64 atomics
+  double *G;           // optional [natoms*G_stride] descriptors (unscaled or scaled)
+  int G_scaled;
+  int G_stride;
+  int desc_only;       // stop after the descriptors
+  // neighbor-list mode (bit-exact lists)
+  int nb_cap;
+  int *nb_count;
+  double *nb_coo;      // [natoms*nb_cap*3]
+  double *nb_dist;
+  int *nb_list;        // 1-based atom index within the structure
+  int *nb_type;
+};
+
+// explicit environments (per-atom C-ABI entry points): neighbors given
+struct EnvDev {
+  int nenv;
+  const int *env_start;   // [nenv+1]
+  const int *env_type;    // [nenv] 1-based central type
+  const double *coo_i;    // [3*nenv]
+  const double *coo_j;    // [3*total]
+  const int *type_j;      // [total]
+  double *E;              // [nenv]
+  double *dF_i;           // [3*nenv] force on the central atom
+  double *dF_j;           // [3*total] force on each neighbor entry
+};

@@ -1,0 +1,593 @@
+// atom_cheb.hip -- per-atom kernel for Chebyshev (AUC) descriptors.
+//
+// Persistent workgroups of 4 wavefronts; each wavefront evaluates one
+// central atom at a time:
+//   1. neighbors  : compact list of k_search (structures) or the caller's
+//                   list (per-atom entry points)
+//   2. descriptors: sfb_eval (src/ext/sfbasis.f90:440-571) -- radial terms
+//                   over lanes = pairs, angular terms over lanes = triplets;
+//                   Chebyshev recurrences in FP32 registers; the per-lane
+//                   partial sums are combined by a reduce-scatter butterfly
+//                   (first level FP32, then FP64) that leaves feature f in
+//                   lane f
+//   3. normalize  : stp_normalize (src/sfsetup.f90:985-1035) in FP64
+//   4. network    : ff_eval / ff_deriv (src/ext/feedforward.f90:586-757),
+//                   lanes = nodes of a layer, weights of every species held
+//                   in LDS for the whole workgroup
+//   5. forces     : F_j -= E_scale dy/dG dG/dR_j (src/aenet.f90:471-477):
+//                   pair and triplet derivatives contracted with dE/dG in
+//                   FP32, per-neighbor sums in LDS, FP64 atomics into F
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "aenet_amd_internal.h"
+#include "kernels.h"
+#include "kernels_common.h"
+
+namespace aenet_amd {
+
+namespace {
+
+constexpr float EPS_R = 1.0e-12f;
+constexpr int MAX_WAVES = 4;  // wavefronts per workgroup (fewer when LDS is short)
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void activate(int f, float x, float &y, float &dy) {
+  // ff_activate (src/ext/feedforward.f90:932-969)
+  switch (f) {
+    case 0: y = x; dy = 1.0f; break;
+    case 1: y = tanhf(x); dy = 1.0f - y * y; break;
+    case 2: y = 1.0f / (1.0f + expf(-x)); dy = y * (1.0f - y); break;
+    case 3: { float t = tanhf(0.666666666666667f * x); y = 1.7159f * t; dy = 1.7159f * (1.0f - t * t) * 0.666666666666667f; } break;
+    case 4: { float t = tanhf(0.666666666666667f * x); y = 1.7159f * t + 0.1f * x; dy = 1.7159f * (1.0f - t * t) * 0.666666666666667f + 0.1f; } break;
+    default: y = 0.0f; dy = 0.0f;
+  }
+}
+
+// cutoff function and its derivative (sfb_fc, sfb_fc_d1,
+// src/ext/sfbasis.f90:719-752)
+__device__ __forceinline__ void cutoff(float r, float Rc, float inv_Rc, float &fc, float &dfc) {
+  if (r >= Rc) {
+    fc = 0.0f;
+    dfc = 0.0f;
+  } else {
+    float sp, cp;
+    sincospif(r * inv_Rc, &sp, &cp);
+    fc = 0.5f * (cp + 1.0f);
+    dfc = -0.5f * 3.14159265358979f * inv_Rc * sp;
+  }
+}
+
+// triangular index p -> (a, b), 0 <= a < b < n, row-major
+__device__ __forceinline__ void tri_decode(int p, int n, int &a, int &b) {
+  float nn = (float)(2 * n - 1);
+  int r = (int)((nn - sqrtf(nn * nn - 8.0f * (float)p)) * 0.5f);
+  if (r < 0) r = 0;
+  if (r > n - 2) r = n - 2;
+  while (r > 0 && r * n - r * (r + 1) / 2 > p) --r;
+  while ((r + 1) * n - (r + 1) * (r + 2) / 2 <= p) ++r;
+  a = r;
+  b = p - (r * n - r * (r + 1) / 2) + r + 1;
+}
+
+// Reduce-scatter of 64 per-lane values over the wavefront: on return lane
+// f holds the sum over all lanes of v[f].  First level in FP32 (pairs of
+// partial sums), the remaining five levels in FP64.
+__device__ __forceinline__ double reduce_scatter64(float (&v)[64]) {
+  const int lane = lane_id();
+  float h[32];
+#pragma unroll
+  for (int k = 0; k < 32; ++k) {
+    bool up = lane & 32;
+    float send = up ? v[k] : v[k + 32];
+    float keep = up ? v[k + 32] : v[k];
+    h[k] = keep + __shfl_xor(send, 32, WAVE);
+  }
+  double d[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    bool up = lane & 16;
+    double send = up ? (double)h[k] : (double)h[k + 16];
+    double keep = up ? (double)h[k + 16] : (double)h[k];
+    d[k] = keep + __shfl_xor(send, 16, WAVE);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    bool up = lane & 8;
+    double send = up ? d[k] : d[k + 8];
+    double keep = up ? d[k + 8] : d[k];
+    d[k] = keep + __shfl_xor(send, 8, WAVE);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    bool up = lane & 4;
+    double send = up ? d[k] : d[k + 4];
+    double keep = up ? d[k + 4] : d[k];
+    d[k] = keep + __shfl_xor(send, 4, WAVE);
+  }
+#pragma unroll
+  for (int k = 0; k < 2; ++k) {
+    bool up = lane & 2;
+    double send = up ? d[k] : d[k + 2];
+    double keep = up ? d[k + 2] : d[k];
+    d[k] = keep + __shfl_xor(send, 2, WAVE);
+  }
+  {
+    bool up = lane & 1;
+    double send = up ? d[0] : d[1];
+    double keep = up ? d[1] : d[0];
+    d[0] = keep + __shfl_xor(send, 1, WAVE);
+  }
+  return d[0];
+}
+
+}  // namespace
+
+// Per-wavefront LDS region, laid out after the weights in dynamic shared
+// memory; cap = neighbor capacity, nsf = max features.
+struct WaveLds {
+  float *R;      // [3*cap]
+  float *r;      // [cap]
+  int *t;        // [cap] neighbor types
+  int *j;        // [cap] neighbor ids (global atom or environment entry)
+  int *ang;      // [cap] angular neighbor list
+  float *fs;     // [3*cap] per-neighbor forces
+  float *gp;     // [nsf] normalized descriptors
+  float *coef;   // [nsf] dE/dG
+  float *d0;     // [dmax] back-propagation buffers, dmax = max(nsf, 64)
+  float *d1;     // [dmax]
+  float *act;    // [nl*64] activations of layer l at act[64*l]
+  float *dact;   // [nl*64]
+};
+
+struct ChebLaunch {
+  int natoms;    // number of central atoms (structures) or environments
+  int cap;       // neighbor capacity of the LDS region
+  int nsf;       // max features
+  int nl;        // max layers
+  int wsize;     // total weights
+  int w_in_lds;  // weights staged in LDS
+  int waves;     // wavefronts per workgroup
+};
+
+__host__ __device__ inline int dmax_words(int nsf) {
+  return nsf > AENET_AMD_MAX_HIDDEN ? nsf : AENET_AMD_MAX_HIDDEN;
+}
+
+__host__ __device__ inline int wave_lds_words(const ChebLaunch &P) {
+  return 10 * P.cap + 2 * P.nsf + 2 * dmax_words(P.nsf) + 2 * P.nl * AENET_AMD_MAX_HIDDEN;
+}
+
+__device__ inline WaveLds carve(float *base, const ChebLaunch &P) {
+  const int cap = P.cap, nsf = P.nsf, dm = dmax_words(P.nsf);
+  WaveLds w;
+  w.R = base;
+  w.r = w.R + 3 * cap;
+  w.t = reinterpret_cast<int *>(w.r + cap);
+  w.j = w.t + cap;
+  w.ang = w.j + cap;
+  w.fs = reinterpret_cast<float *>(w.ang + cap);
+  w.gp = w.fs + 3 * cap;
+  w.coef = w.gp + nsf;
+  w.d0 = w.coef + nsf;
+  w.d1 = w.d0 + dm;
+  w.act = w.d1 + dm;
+  w.dact = w.act + P.nl * AENET_AMD_MAX_HIDDEN;
+  return w;
+}
+
+// MODE 0: structures (compact lists, FP64 atomic force scatter)
+// MODE 1: explicit environments (per-neighbor forces out)
+template <int RN, int AN, int MODE, bool FORCES>
+__global__ __launch_bounds__(64 * MAX_WAVES) void k_atom_cheb(ModelDev md, BatchDev b, NbDev nbl, OutDev o, EnvDev env,
+                                                          ChebLaunch P) {
+  static_assert(2 * (RN + AN) <= 64, "features of one chunk");
+  extern __shared__ float lds[];
+  const int lane = lane_id();
+  const int wave = threadIdx.x / WAVE;
+  const int nw = P.waves;
+  // weights of every species, once per workgroup
+  const float *Wsrc = md.W;
+  if (P.w_in_lds) {
+    for (int k = threadIdx.x; k < P.wsize; k += 64 * nw) lds[k] = md.W[k];
+    __syncthreads();
+    Wsrc = lds;
+  }
+  const int wsz = P.w_in_lds ? (P.wsize + 3) / 4 * 4 : 0;
+  WaveLds S = carve(lds + wsz + wave * wave_lds_words(P), P);
+
+  for (int g = blockIdx.x * nw + wave; g < P.natoms; g += gridDim.x * nw) {
+    wave_sync();
+    // ------------------------------------------------------------ 1
+    int ti, n;
+    if (MODE == 0) {
+      ti = b.types[g];
+      n = min(nbl.count[g], P.cap);
+      const float4 *R4 = reinterpret_cast<const float4 *>(nbl.R) + (size_t)g * nbl.cap;
+      const int *J = nbl.j + (size_t)g * nbl.cap;
+      for (int q = lane; q < n; q += WAVE) {
+        float4 v = R4[q];
+        int j = J[q];
+        S.R[3 * q] = v.x;
+        S.R[3 * q + 1] = v.y;
+        S.R[3 * q + 2] = v.z;
+        S.r[q] = v.w;
+        S.j[q] = j;
+        S.t[q] = b.types[j];
+      }
+    } else {
+      ti = env.env_type[g];
+      int e0 = env.env_start[g];
+      n = env.env_start[g + 1] - e0;
+      if (n > P.cap) {
+        if (lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_NB_OVERFLOW);
+        n = P.cap;
+      }
+      const double ci[3] = {env.coo_i[3 * g], env.coo_i[3 * g + 1], env.coo_i[3 * g + 2]};
+      for (int q = lane; q < n; q += WAVE) {
+        double c[3];
+        for (int k = 0; k < 3; ++k) c[k] = env.coo_j[3 * (size_t)(e0 + q) + k] - ci[k];
+        S.R[3 * q] = (float)c[0];
+        S.R[3 * q + 1] = (float)c[1];
+        S.R[3 * q + 2] = (float)c[2];
+        S.r[q] = (float)sqrt(dist2_exact(c));
+        S.j[q] = e0 + q;
+        S.t[q] = env.type_j[e0 + q];
+      }
+    }
+    ti = uniform(ti);
+    n = uniform(n);
+    if (ti < 1 || ti > md.ntypes) {
+      if (lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_BAD_TYPE);
+      continue;
+    }
+    wave_sync();
+    const SpeciesDev &sp = md.species[ti - 1];
+    const int rN = sp.r_N, aN = sp.a_N, nsf = sp.nsf;
+    const bool multi = sp.multi != 0;
+    const float rRc = sp.r_Rc, aRc = sp.a_Rc;
+    const float inv_rRc = 1.0f / rRc;
+    const float inv_aRc = aRc > 0.0f ? 1.0f / aRc : 0.0f;
+
+    // ------------------------------------------------------------ 2
+    // accumulators in template layout: [radial RN | angular AN |
+    // radial*spin RN | angular*spin AN | padding]
+    float acc[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) acc[k] = 0.0f;
+    for (int q = lane; q < n; q += WAVE) {
+      float r = S.r[q];
+      if (r <= rRc && r > EPS_R) {
+        float fc, dfc;
+        cutoff(r, rRc, inv_rRc, fc, dfc);
+        float x = 2.0f * r * inv_rRc - 1.0f;
+        float sfc = sp.spin[S.t[q] - 1] * fc;
+        float Tm = 1.0f, T = x;
+        acc[0] += fc;
+        acc[RN + AN] += sfc;
+#pragma unroll
+        for (int k = 1; k < RN; ++k) {
+          acc[k] += fc * T;
+          acc[RN + AN + k] += sfc * T;
+          float Tn = 2.0f * x * T - Tm;
+          Tm = T;
+          T = Tn;
+        }
+      }
+    }
+    // angular neighbor list (order preserved)
+    int na = 0;
+    for (int base = 0; base < n; base += WAVE) {
+      int q = base + lane;
+      bool in = false;
+      if (q < n) {
+        float r = S.r[q];
+        in = r <= aRc && r >= EPS_R;
+      }
+      unsigned long long m = wave_ballot(in);
+      if (in) S.ang[na + mask_prefix(m)] = q;
+      na += popc64(m);
+    }
+    wave_sync();
+    const int ntrip = na * (na - 1) / 2;
+    for (int p = lane; p < ntrip; p += WAVE) {
+      int a, c;
+      tri_decode(p, na, a, c);
+      int qj = S.ang[a], qk = S.ang[c];
+      float rj = S.r[qj], rk = S.r[qk];
+      float cosv = (S.R[3 * qj] * S.R[3 * qk] + S.R[3 * qj + 1] * S.R[3 * qk + 1] + S.R[3 * qj + 2] * S.R[3 * qk + 2]) /
+                   (rj * rk);
+      float fcj, dfcj, fck, dfck;
+      cutoff(rj, aRc, inv_aRc, fcj, dfcj);
+      cutoff(rk, aRc, inv_aRc, fck, dfck);
+      float w = fcj * fck;
+      float sw = sp.spin[S.t[qj] - 1] * sp.spin[S.t[qk] - 1] * w;
+      float Tm = 1.0f, T = cosv;
+      acc[RN] += w;
+      acc[2 * RN + AN] += sw;
+#pragma unroll
+      for (int k = 1; k < AN; ++k) {
+        acc[RN + k] += w * T;
+        acc[2 * RN + AN + k] += sw * T;
+        float Tn = 2.0f * cosv * T - Tm;
+        Tm = T;
+        T = Tn;
+      }
+    }
+    double G = reduce_scatter64(acc);
+    // lane f holds template feature f; map it to the species' layout
+    {
+      int f = -1;
+      if (lane < RN) {
+        if (lane < rN) f = sp.r_i1 + lane;
+      } else if (lane < RN + AN) {
+        if (lane - RN < aN) f = sp.a_i1 + lane - RN;
+      } else if (lane < 2 * RN + AN) {
+        if (multi && lane - RN - AN < rN) f = sp.r_i2 + lane - RN - AN;
+      } else if (lane < 2 * (RN + AN)) {
+        if (multi && lane - 2 * RN - AN < aN) f = sp.a_i2 + lane - 2 * RN - AN;
+      }
+      if (f >= 0) {
+        // ------------------------------------------------------ 3
+        double Gs = md.norm_scale[sp.norm_off + f] * (G - md.norm_avg[sp.norm_off + f]);
+        S.gp[f] = (float)Gs;
+        if (o.G) o.G[(size_t)g * o.G_stride + f] = o.G_scaled ? Gs : G;
+      }
+    }
+    wave_sync();
+    if (o.desc_only) continue;
+
+    // ------------------------------------------------------------ 4
+    const int nl = sp.nlayers;
+    const float *in = S.gp;
+    for (int l = 0; l + 1 < nl; ++l) {
+      const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
+      const float *Wl = Wsrc + sp.w_off[l];
+      if (lane < n2) {
+        float z0 = Wl[n1 * n2 + lane], z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;  // bias column
+        int c = 0;
+        for (; c + 3 < n1; c += 4) {
+          z0 += Wl[c * n2 + lane] * in[c];
+          z1 += Wl[(c + 1) * n2 + lane] * in[c + 1];
+          z2 += Wl[(c + 2) * n2 + lane] * in[c + 2];
+          z3 += Wl[(c + 3) * n2 + lane] * in[c + 3];
+        }
+        for (; c < n1; ++c) z0 += Wl[c * n2 + lane] * in[c];
+        float y, dy;
+        activate(sp.fa[l + 1], (z0 + z1) + (z2 + z3), y, dy);
+        S.act[(l + 1) * AENET_AMD_MAX_HIDDEN + lane] = y;
+        S.dact[(l + 1) * AENET_AMD_MAX_HIDDEN + lane] = dy;
+      }
+      wave_sync();
+      in = S.act + (l + 1) * AENET_AMD_MAX_HIDDEN;
+    }
+    const float yout = S.act[(nl - 1) * AENET_AMD_MAX_HIDDEN];
+    float *dcur = S.d0;
+    float *dnext = S.d1;
+    if (lane == 0) dcur[0] = S.dact[(nl - 1) * AENET_AMD_MAX_HIDDEN];
+    wave_sync();
+    for (int l = nl - 2; l >= 0; --l) {
+      const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
+      const float *Wl = Wsrc + sp.w_off[l];
+      for (int c = lane; c < n1; c += WAVE) {
+        const float *row = Wl + c * n2;
+        float s0 = 0.0f, s1 = 0.0f;
+        int hh = 0;
+        for (; hh + 1 < n2; hh += 2) {
+          s0 += row[hh] * dcur[hh];
+          s1 += row[hh + 1] * dcur[hh + 1];
+        }
+        if (hh < n2) s0 += row[hh] * dcur[hh];
+        float s = s0 + s1;
+        dnext[c] = l > 0 ? s * S.dact[l * AENET_AMD_MAX_HIDDEN + c] : s;
+      }
+      wave_sync();
+      float *t = dcur;
+      dcur = dnext;
+      dnext = t;
+    }
+    const float Es = (float)sp.E_scale;
+    for (int f = lane; f < nsf; f += WAVE) S.coef[f] = Es * dcur[f] * (float)md.norm_scale[sp.norm_off + f];
+    if (lane == 0) {
+      double E = sp.E_scale * (double)yout + sp.E_shift;
+      E = E + sp.E_atom;
+      if (MODE == 0)
+        o.E_atom[g] = E;
+      else
+        env.E[g] = E;
+    }
+    if (!FORCES) continue;
+    wave_sync();
+
+    // ------------------------------------------------------------ 5
+    float *fs = S.fs;
+    const float *cr1 = S.coef + sp.r_i1, *cr2 = S.coef + sp.r_i2;
+    const float *ca1 = S.coef + sp.a_i1, *ca2 = S.coef + sp.a_i2;
+    for (int q = lane; q < n; q += WAVE) {
+      float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+      float r = S.r[q];
+      if (r <= rRc && r > EPS_R) {
+        float fc, dfc;
+        cutoff(r, rRc, inv_rRc, fc, dfc);
+        float x = 2.0f * r * inv_rRc - 1.0f;
+        float sj = multi ? sp.spin[S.t[q] - 1] : 0.0f;
+        float fcx = fc * 2.0f * inv_rRc;
+        // sigma = sum_n a_n (dfc T_n + fc T'_n dx/dr), T'_n = n U_{n-1}
+        float sig = (cr1[0] + sj * cr2[0]) * dfc;
+        float Tm = 1.0f, T = x, Um = 0.0f, U = 1.0f;
+#pragma unroll
+        for (int k = 1; k < RN; ++k) {
+          if (k < rN) {
+            float a = cr1[k] + sj * cr2[k];
+            sig += a * (dfc * T + fcx * (float)k * U);
+          }
+          float Tn = 2.0f * x * T - Tm;
+          float Un = 2.0f * x * U - Um;
+          Tm = T;
+          T = Tn;
+          Um = U;
+          U = Un;
+        }
+        float s = sig / r;
+        gx = s * S.R[3 * q];
+        gy = s * S.R[3 * q + 1];
+        gz = s * S.R[3 * q + 2];
+      }
+      fs[3 * q] = gx;
+      fs[3 * q + 1] = gy;
+      fs[3 * q + 2] = gz;
+    }
+    wave_sync();
+    for (int p = lane; p < ntrip; p += WAVE) {
+      int a, c;
+      tri_decode(p, na, a, c);
+      int qj = S.ang[a], qk = S.ang[c];
+      float rj = S.r[qj], rk = S.r[qk];
+      float Rj[3] = {S.R[3 * qj], S.R[3 * qj + 1], S.R[3 * qj + 2]};
+      float Rk[3] = {S.R[3 * qk], S.R[3 * qk + 1], S.R[3 * qk + 2]};
+      float ijk = 1.0f / (rj * rk);
+      float cosv = (Rj[0] * Rk[0] + Rj[1] * Rk[1] + Rj[2] * Rk[2]) * ijk;
+      float fcj, dfcj, fck, dfck;
+      cutoff(rj, aRc, inv_aRc, fcj, dfcj);
+      cutoff(rk, aRc, inv_aRc, fck, dfck);
+      float w = fcj * fck;
+      float ss = multi ? sp.spin[S.t[qj] - 1] * sp.spin[S.t[qk] - 1] : 0.0f;
+      // A = sum b_n T_n(cos), B = sum b_n T'_n(cos)
+      float A = ca1[0] + ss * ca2[0];
+      float Bd = 0.0f;
+      float Tm = 1.0f, T = cosv, Um = 0.0f, U = 1.0f;
+#pragma unroll
+      for (int k = 1; k < AN; ++k) {
+        if (k < aN) {
+          float bk = ca1[k] + ss * ca2[k];
+          A += bk * T;
+          Bd += bk * (float)k * U;
+        }
+        float Tn = 2.0f * cosv * T - Tm;
+        float Un = 2.0f * cosv * U - Um;
+        Tm = T;
+        T = Tn;
+        Um = U;
+        U = Un;
+      }
+      float wB = w * Bd;
+      float aj = A * dfcj * fck / rj, ak = A * fcj * dfck / rk;
+      float cj = cosv / (rj * rj), ck = cosv / (rk * rk);
+#pragma unroll
+      for (int x = 0; x < 3; ++x) {
+        float gj = aj * Rj[x] + wB * (Rk[x] * ijk - cj * Rj[x]);
+        float gk = ak * Rk[x] + wB * (Rj[x] * ijk - ck * Rk[x]);
+        atomicAdd(&fs[3 * qj + x], gj);
+        atomicAdd(&fs[3 * qk + x], gk);
+      }
+    }
+    wave_sync();
+    // F_j -= dE_i/dR_j ; F_i += sum_j dE_i/dR_j
+    double own[3] = {0.0, 0.0, 0.0};
+    for (int q = lane; q < n; q += WAVE) {
+      double gx = fs[3 * q], gy = fs[3 * q + 1], gz = fs[3 * q + 2];
+      own[0] += gx;
+      own[1] += gy;
+      own[2] += gz;
+      if (MODE == 0) {
+        int j = S.j[q];
+        unsafeAtomicAdd(&o.F[3 * (size_t)j], -gx);
+        unsafeAtomicAdd(&o.F[3 * (size_t)j + 1], -gy);
+        unsafeAtomicAdd(&o.F[3 * (size_t)j + 2], -gz);
+      } else {
+        size_t e = (size_t)S.j[q];
+        env.dF_j[3 * e] = -gx;
+        env.dF_j[3 * e + 1] = -gy;
+        env.dF_j[3 * e + 2] = -gz;
+      }
+    }
+    for (int x = 0; x < 3; ++x) own[x] = wave_sum(own[x]);
+    if (lane == 0) {
+      if (MODE == 0) {
+        unsafeAtomicAdd(&o.F[3 * (size_t)g], own[0]);
+        unsafeAtomicAdd(&o.F[3 * (size_t)g + 1], own[1]);
+        unsafeAtomicAdd(&o.F[3 * (size_t)g + 2], own[2]);
+      } else {
+        env.dF_i[3 * g] = own[0];
+        env.dF_i[3 * g + 1] = own[1];
+        env.dF_i[3 * g + 2] = own[2];
+      }
+    }
+    wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------
+// launch: template dispatch on the (radial, angular) term counts
+// ---------------------------------------------------------------------
+
+template <int RN, int AN, int MODE, bool F>
+static void launch_one(const ModelDev &md, const BatchDev &b, const NbDev &nb, const OutDev &o, const EnvDev &e,
+                       ChebLaunch P, hipStream_t stream) {
+  const DeviceLimits &dl = device_limits();
+  // LDS: the weights (when they fit beside one wavefront's region) and one
+  // region per wavefront; as many wavefronts per workgroup as fit
+  const size_t wave_bytes = 4 * (size_t)wave_lds_words(P);
+  size_t wbytes = 4 * (size_t)((P.wsize + 3) / 4 * 4);
+  P.w_in_lds = P.wsize <= 16384 && wbytes + wave_bytes <= dl.lds_per_block;
+  if (!P.w_in_lds) wbytes = 0;
+  P.waves = (int)std::min<size_t>(MAX_WAVES, (dl.lds_per_block - wbytes) / wave_bytes);
+  if (P.waves < 1) {
+    // cannot happen within the capacity limits; refuse rather than overrun LDS
+    raise_flag(b.flags, AENET_AMD_FLAG_NB_OVERFLOW, stream);
+    return;
+  }
+  const size_t lds = wbytes + (size_t)P.waves * wave_bytes;
+  // persistent grid: as many workgroups as fit on every CU at once
+  int per_cu = (int)(dl.lds_per_cu / lds);
+  per_cu = std::max(1, std::min(per_cu, 32 / P.waves));
+  int grid = std::min((P.natoms + P.waves - 1) / P.waves, dl.cus * per_cu);
+  grid = std::max(grid, 1);
+  hipLaunchKernelGGL((k_atom_cheb<RN, AN, MODE, F>), dim3(grid), dim3(64 * P.waves), lds, stream, md, b, nb, o, e,
+                     P);
+}
+
+template <int RN, int AN>
+static void launch_cheb(int mode, bool forces, const ModelDev &md, const BatchDev &b, const NbDev &nb,
+                        const OutDev &o, const EnvDev &e, ChebLaunch P, hipStream_t stream) {
+  if (mode == 0) {
+    if (forces)
+      launch_one<RN, AN, 0, true>(md, b, nb, o, e, P, stream);
+    else
+      launch_one<RN, AN, 0, false>(md, b, nb, o, e, P, stream);
+  } else {
+    if (forces)
+      launch_one<RN, AN, 1, true>(md, b, nb, o, e, P, stream);
+    else
+      launch_one<RN, AN, 1, false>(md, b, nb, o, e, P, stream);
+  }
+}
+
+void launch_atom_cheb(int rN, int aN, int mode, bool forces, const AtomSizes &z, const ModelDev &md,
+                      const BatchDev &b, const NbDev &nb, const OutDev &o, const EnvDev &e, hipStream_t stream) {
+  if (z.natoms <= 0) return;
+  ChebLaunch P;
+  P.natoms = z.natoms;
+  P.cap = z.cap;
+  P.nsf = z.nsf_max;
+  P.nl = std::max(z.nl_max, 2);
+  P.wsize = z.wsize;
+  P.w_in_lds = 0;
+  P.waves = 1;
+  if (rN <= 9 && aN <= 1)
+    launch_cheb<9, 1>(mode, forces, md, b, nb, o, e, P, stream);
+  else if (rN <= 17 && aN <= 5)
+    launch_cheb<17, 5>(mode, forces, md, b, nb, o, e, P, stream);
+  else if (rN <= 19 && aN <= 7)
+    launch_cheb<19, 7>(mode, forces, md, b, nb, o, e, P, stream);
+  else
+    launch_cheb<24, 8>(mode, forces, md, b, nb, o, e, P, stream);
+}
+
+}  // namespace aenet_amd

@@ -1,0 +1,768 @@
+// capi.cpp -- the libaenet C ABI (include/aenet.h) and the batched
+// extensions (include/aenet_amd.h), backed by the gfx950 kernels.
+//
+// Like the reference library (src/aenet.f90:122-132) the potential set is
+// process-global; a mutex serializes calls that use it.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "aenet_capi_status.h"
+#include "engine.h"
+#include "model.h"
+
+using namespace aenet_amd;
+
+extern "C" {
+int AENET_OK = AENET_STATUS_OK;
+int AENET_ERR_INIT = AENET_STATUS_ERR_INIT;
+int AENET_ERR_MALLOC = AENET_STATUS_ERR_MALLOC;
+int AENET_ERR_IO = AENET_STATUS_ERR_IO;
+int AENET_ERR_TYPE = AENET_STATUS_ERR_TYPE;
+int AENET_TYPELEN = 2;
+int AENET_PATHLEN = 1024;
+bool AENET_TRUE = true;
+bool AENET_FALSE = false;
+int aenet_nsf_max = 0;
+int aenet_nnb_max = 0;
+double aenet_Rc_min = 0.0;
+double aenet_Rc_max = 0.0;
+}
+
+namespace {
+
+std::mutex g_mu;
+thread_local std::string g_err;
+
+struct Global {
+  bool init = false;
+  PotSet set;
+  std::unique_ptr<DeviceModel> dev;
+  hipStream_t stream = nullptr;
+  EnvScratch env;
+  // standalone fingerprint basis (aenet_sfb_*)
+  bool sfb_init = false;
+  PotSet sfb_set;
+  std::unique_ptr<DeviceModel> sfb_dev;
+  int sfb_N = 0, sfb_rN = 0;
+  double sfb_rRc = 0.0;
+  // neighbor list state (aenet_nbl_*)
+  bool nbl_init = false;
+  int nbl_natoms = 0, nbl_cap = 0;
+  std::vector<int> nbl_count, nbl_list, nbl_type;
+  std::vector<double> nbl_coo, nbl_dist;
+};
+
+Global &G() {
+  static Global g;
+  return g;
+}
+
+int set_err(int st, const std::string &msg) {
+  g_err = msg;
+  return st;
+}
+
+hipStream_t lib_stream() {
+  Global &g = G();
+  if (!g.stream) {
+    if (hipStreamCreateWithFlags(&g.stream, hipStreamNonBlocking) != hipSuccess) g.stream = nullptr;
+  }
+  return g.stream;
+}
+
+int ensure_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+    return set_err(AENET_STATUS_ERR_DEVICE, "no HIP device available: libaenet_amd runs on AMD GPUs only");
+  return AENET_STATUS_OK;
+}
+
+std::string cname(const char *s) {
+  // TYPELEN+1 characters up to the terminating NUL (src/aenet.f90:184-189)
+  std::string out;
+  for (int i = 0; i < 3 && s && s[i]; ++i) out.push_back(s[i]);
+  if (out.size() > 2) out.resize(2);
+  return out;
+}
+
+void clear_set(Global &g) {
+  g.set = PotSet();
+  g.dev.reset();
+  g.init = false;
+  aenet_nsf_max = 0;
+  aenet_nnb_max = 0;
+  aenet_Rc_min = 0.0;
+  aenet_Rc_max = 0.0;
+}
+
+// one environment through the device kernel; adds forces into F (FP64)
+int eval_env(Global &g, const double *coo_i, int type_i, int n_j, const double *coo_j, const int *type_j,
+             const int *index_j, int index_i, double *F, double *E_i) {
+  if (type_i < 1 || type_i > g.set.ntypes) return set_err(AENET_STATUS_ERR_TYPE, "central type out of range");
+  for (int j = 0; j < n_j; ++j)
+    if (type_j[j] < 1 || type_j[j] > g.set.ntypes)
+      return set_err(AENET_STATUS_ERR_TYPE, "neighbor type out of range");
+  int start[2] = {0, n_j};
+  bool forces = F != nullptr;
+  std::vector<double> dFj(forces ? 3 * (size_t)n_j : 0);
+  double dFi[3] = {0, 0, 0};
+  std::string err;
+  int st = g.env.eval(*g.dev, 1, start, &type_i, coo_i, coo_j, type_j, forces, E_i, dFi, dFj.data(), nullptr,
+                      0, lib_stream(), err);
+  if (st) return set_err(st, err);
+  if (forces) {
+    for (int x = 0; x < 3; ++x) F[3 * (size_t)(index_i - 1) + x] += dFi[x];
+    for (int j = 0; j < n_j; ++j)
+      for (int x = 0; x < 3; ++x) F[3 * (size_t)(index_j[j] - 1) + x] += dFj[3 * (size_t)j + x];
+  }
+  return AENET_STATUS_OK;
+}
+
+}  // namespace
+
+struct aenet_amd_batch {
+  Batch batch;
+};
+
+extern "C" {
+
+const char *aenet_amd_last_error(void) { return g_err.c_str(); }
+
+// ---------------------------------------------------------------------
+// aenet_init / aenet_final / aenet_load_potential (src/aenet.f90:140-331)
+// ---------------------------------------------------------------------
+
+void aenet_init(int ntypes, char *atom_types[], int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  *stat = AENET_STATUS_OK;
+  if (g.init) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "aenet_init: already initialized");
+    return;
+  }
+  if (ntypes < 1 || ntypes > AENET_AMD_MAX_TYPES) {
+    *stat = set_err(AENET_STATUS_ERR_MALLOC, "aenet_init: 1 to 8 species are supported");
+    return;
+  }
+  g.set = PotSet();
+  g.set.ntypes = ntypes;
+  for (int i = 0; i < ntypes; ++i) g.set.names.push_back(cname(atom_types[i]));
+  g.set.pots.assign(ntypes, PotHost());
+  g.init = true;
+}
+
+void aenet_final(int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  *stat = AENET_STATUS_OK;
+  if (g.init) clear_set(g);
+}
+
+void aenet_print_info(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  if (!g.init) {
+    std::printf(" Nothing to print. AenetLib is not initialized.\n");
+    return;
+  }
+  for (int t = 0; t < g.set.ntypes; ++t) {
+    const PotHost &p = g.set.pots[t];
+    if (!p.init) continue;
+    std::printf(" Atomic species : %s\n File name      : %s\n\n", p.stp.atomtype.c_str(), p.file.c_str());
+    std::printf(" Network: %d layers (", p.net.nlayers);
+    for (int l = 0; l < p.net.nlayers; ++l) std::printf(l ? "-%d" : "%d", p.net.nnodes[l]);
+    std::printf("), %d weights\n", p.net.Wsize);
+    std::printf(" Descriptor: %s, %d functions, Rc_min %.3f, Rc_max %.3f\n", p.stp.sftype.c_str(), p.stp.nsf,
+                p.stp.Rc_min, p.stp.Rc_max);
+    std::printf(" E_scale %.10g  E_shift %.10g  E_atom %.10g\n\n", p.E_scale, p.E_shift, p.E_atom);
+  }
+  std::fflush(stdout);
+}
+
+void aenet_load_potential(int type_id, char *filename, int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  *stat = AENET_STATUS_OK;
+  if (!filename || std::strlen(filename) > 1024) {
+    *stat = set_err(AENET_STATUS_ERR_MALLOC, "aenet_load_potential: path longer than 1024 characters");
+    return;
+  }
+  if (!g.init) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "aenet_load_potential: call aenet_init first");
+    return;
+  }
+  if (type_id < 1 || type_id > g.set.ntypes) {
+    *stat = set_err(AENET_STATUS_ERR_TYPE, "aenet_load_potential: type id out of range");
+    return;
+  }
+  struct stat sb;
+  if (::stat(filename, &sb) != 0) {
+    *stat = set_err(AENET_STATUS_ERR_IO, std::string("aenet_load_potential: file not found: ") + filename);
+    return;
+  }
+  std::string err;
+  PotHost pot;
+  int st = read_nn(filename, pot, err);
+  if (st) {
+    *stat = set_err(st, err);
+    return;
+  }
+  g.set.pots[type_id - 1] = pot;
+  g.set.loaded = false;
+  bool all = true;
+  for (auto &p : g.set.pots) all = all && p.init;
+  if (all) {
+    st = g.set.finalize(err);
+    if (st) {
+      *stat = set_err(st, err);
+      return;
+    }
+    if ((st = ensure_device())) {
+      *stat = st;
+      return;
+    }
+    g.dev.reset(new DeviceModel());
+    if (g.dev->upload(g.set, lib_stream())) {
+      *stat = set_err(AENET_STATUS_ERR_DEVICE, "aenet_load_potential: device upload failed");
+      return;
+    }
+    aenet_Rc_min = g.set.Rc_min;
+    aenet_Rc_max = g.set.Rc_max;
+    aenet_nnb_max = g.set.nnb_max;
+    aenet_nsf_max = g.set.nsf_max;
+  }
+}
+
+bool aenet_all_loaded(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  if (!g.init) return true;  // the reference loops over zero types
+  for (auto &p : g.set.pots)
+    if (!p.init) return false;
+  return true;
+}
+
+double aenet_free_atom_energy(int type_id) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  if (!g.init || type_id < 1 || type_id > g.set.ntypes || !g.set.pots[type_id - 1].init) return NAN;
+  return g.set.pots[type_id - 1].E_atom;
+}
+
+// ---------------------------------------------------------------------
+// per-atom evaluation (src/aenet.f90:359-486)
+// ---------------------------------------------------------------------
+
+void aenet_atomic_energy(double coo_i[3], int type_i, int n_j, double coo_j[], int type_j[], double *E_i,
+                         int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  if (!g.init || !g.set.loaded) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "aenet_atomic_energy: potentials not loaded");
+    return;
+  }
+  *stat = eval_env(g, coo_i, type_i, n_j, coo_j, type_j, nullptr, 0, nullptr, E_i);
+}
+
+void aenet_atomic_energy_and_forces(double coo_i[3], int type_i, int index_i, int n_j, double coo_j[],
+                                    int type_j[], int index_j[], int natoms, double *E_i, double F[],
+                                    int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  if (!g.init || !g.set.loaded) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "aenet_atomic_energy_and_forces: potentials not loaded");
+    return;
+  }
+  if (index_i < 1 || index_i > natoms) {
+    *stat = set_err(AENET_STATUS_ERR_TYPE, "atom index out of range");
+    return;
+  }
+  for (int j = 0; j < n_j; ++j)
+    if (index_j[j] < 1 || index_j[j] > natoms) {
+      *stat = set_err(AENET_STATUS_ERR_TYPE, "neighbor index out of range");
+      return;
+    }
+  *stat = eval_env(g, coo_i, type_i, n_j, coo_j, type_j, index_j, index_i, F, E_i);
+}
+
+// geo_type_conv (src/geometry.f90:440-461) over the loaded species
+void aenet_convert_atom_types(int ntypes_in, char *atom_types[], int natoms_in, int type_id_in[],
+                              int type_id_out[], int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  *stat = AENET_STATUS_OK;
+  if (!g.init) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "aenet_convert_atom_types: not initialized");
+    return;
+  }
+  for (int i = 0; i < natoms_in; ++i) {
+    int it = type_id_in[i];
+    int out = 0;
+    if (it >= 1 && it <= ntypes_in) {
+      std::string nm = cname(atom_types[it - 1]);
+      for (int t = 0; t < g.set.ntypes; ++t)
+        if (name_eq(g.set.names[t], nm)) {
+          out = t + 1;
+          break;
+        }
+    }
+    if (out == 0) {
+      *stat = set_err(AENET_STATUS_ERR_TYPE, "aenet_convert_atom_types: unknown species");
+      return;
+    }
+    type_id_out[i] = out;
+  }
+}
+
+// ---------------------------------------------------------------------
+// standalone Chebyshev basis (src/aenet.f90:652-799)
+// ---------------------------------------------------------------------
+
+void aenet_sfb_init(int ntypes, char *atom_types[], int radial_order, int angular_order, double radial_Rc,
+                    double angular_Rc, int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  *stat = AENET_STATUS_OK;
+  if (g.sfb_init) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "aenet_sfb_init: already initialized");
+    return;
+  }
+  if (ntypes < 1 || ntypes > AENET_AMD_MAX_TYPES || radial_order < 0 || radial_order > 31 ||
+      angular_order < 0 || angular_order > 15 || radial_Rc <= 0.0) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_sfb_init: unsupported basis parameters");
+    return;
+  }
+  int st = ensure_device();
+  if (st) {
+    *stat = st;
+    return;
+  }
+  // a potential set with one network-free species per type and the
+  // descriptor of new_SFBasis (src/ext/sfbasis.f90:83-151)
+  PotSet ps;
+  ps.ntypes = ntypes;
+  std::vector<std::string> names;
+  for (int i = 0; i < ntypes; ++i) names.push_back(cname(atom_types[i]));
+  ps.names = names;
+  int nsf = (radial_order + 1 + angular_order + 1) * (ntypes > 1 ? 2 : 1);
+  for (int t = 0; t < ntypes; ++t) {
+    PotHost p;
+    p.init = true;
+    p.net.nlayers = 2;
+    p.net.nnodes = {nsf, 1};
+    p.net.fa = {0, 0};
+    p.net.iw = {0, nsf + 1};
+    p.net.iv = {0, nsf + 1};
+    p.net.Wsize = nsf + 1;
+    p.net.W.assign(nsf + 1, 0.0);
+    p.stp.atomtype = names[t];
+    p.stp.envtypes = names;
+    p.stp.sftype = "Chebyshev";
+    p.stp.nsf = nsf;
+    p.stp.nsfparam = 4;
+    p.stp.sfparam.assign(4 * (size_t)nsf, 0.0);
+    p.stp.sfparam[0] = radial_Rc;
+    p.stp.sfparam[1] = radial_order;
+    p.stp.sfparam[2] = angular_Rc;
+    p.stp.sfparam[3] = angular_order;
+    p.stp.Rc_min = 1.0;
+    p.stp.Rc_max = std::max(radial_Rc, angular_Rc);
+    p.stp.avg.assign(nsf, 0.0);
+    p.stp.cov.assign(nsf, 1.0);
+    p.stp.sf.assign(nsf, 0);
+    p.stp.sfenv.assign(2 * (size_t)nsf, 0);
+    ps.pots.push_back(p);
+  }
+  std::string err;
+  st = ps.finalize(err);
+  if (st) {
+    *stat = set_err(st, err);
+    return;
+  }
+  g.sfb_set = ps;
+  g.sfb_dev.reset(new DeviceModel());
+  if (g.sfb_dev->upload(g.sfb_set, lib_stream())) {
+    *stat = set_err(AENET_STATUS_ERR_DEVICE, "aenet_sfb_init: device upload failed");
+    return;
+  }
+  g.sfb_N = nsf;
+  g.sfb_rN = radial_order + 1;
+  g.sfb_rRc = radial_Rc;
+  g.sfb_init = true;
+}
+
+void aenet_sfb_final(int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  *stat = AENET_STATUS_OK;
+  if (!g.sfb_init) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "aenet_sfb_final: not initialized");
+    return;
+  }
+  g.sfb_dev.reset();
+  g.sfb_set = PotSet();
+  g.sfb_init = false;
+}
+
+int aenet_sfb_nvalues(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return G().sfb_init ? G().sfb_N : 0;
+}
+
+void aenet_sfb_eval(int itype0, double coo0[3], int nat, int itype1[], double coo1[], int nv, double values[],
+                    int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  *stat = AENET_STATUS_OK;
+  if (!g.sfb_init) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "aenet_sfb_eval: not initialized");
+    return;
+  }
+  if (nv != g.sfb_N) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_sfb_eval: wrong number of basis functions");
+    return;
+  }
+  int nt = g.sfb_set.ntypes;
+  if (itype0 < 1 || itype0 > nt) {
+    *stat = set_err(AENET_STATUS_ERR_TYPE, "aenet_sfb_eval: type out of range");
+    return;
+  }
+  for (int j = 0; j < nat; ++j)
+    if (itype1[j] < 1 || itype1[j] > nt) {
+      *stat = set_err(AENET_STATUS_ERR_TYPE, "aenet_sfb_eval: type out of range");
+      return;
+    }
+  int start[2] = {0, nat};
+  std::string err;
+  int st = g.env.eval(*g.sfb_dev, 1, start, &itype0, coo0, coo1, itype1, false, nullptr, nullptr, nullptr,
+                      values, nv, lib_stream(), err);
+  if (st) *stat = set_err(st, err);
+}
+
+// sfb_reconstruct_radial (src/ext/sfbasis.f90:581-637): the radial
+// distribution function from the first r_N expansion coefficients
+void aenet_sfb_reconstruct_radial(int nv, double values[], int nx, double x[], double y[], int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  *stat = AENET_STATUS_OK;
+  if (!g.sfb_init) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "aenet_sfb_reconstruct_radial: not initialized");
+    return;
+  }
+  (void)nv;
+  const double PI = 3.14159265358979, PI_INV = 1.0 / PI;
+  const double Rc = g.sfb_rRc;
+  const int rN = g.sfb_rN;
+  double dx = Rc / (double)(nx - 1);
+  for (int i = 0; i < nx; ++i) x[i] = (double)i * dx;
+  std::vector<double> f(rN);
+  for (int i = 0; i < nx - 1; ++i) {
+    double xx = (2.0 * x[i] - 0.0 - Rc) / (Rc - 0.0);
+    f[0] = 1.0;
+    if (rN > 1) f[1] = xx;
+    for (int k = 2; k < rN; ++k) f[k] = 2.0 * xx * f[k - 1] - f[k - 2];
+    double r = x[i] / Rc;
+    double w = 0.5 / std::sqrt(r - r * r);
+    for (int k = 0; k < rN; ++k) f[k] = f[k] * w * PI_INV;
+    f[0] = 0.5 * f[0];
+    double acc = 0.0;
+    for (int k = rN - 1; k >= 0; --k) acc = acc + values[k] * f[k];
+    y[i] = acc;
+  }
+  if (nx > 0) y[nx - 1] = 0.0;
+}
+
+// ---------------------------------------------------------------------
+// neighbor lists (src/aenet.f90:571-639)
+// ---------------------------------------------------------------------
+
+void aenet_nbl_init(double latt_vec[9], int natoms, int atom_types[], double coords[], bool cartesian, bool pbc) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  g.nbl_init = false;
+  if (!g.init || !g.set.loaded || natoms < 1) {
+    set_err(AENET_STATUS_ERR_INIT, "aenet_nbl_init: potentials not loaded");
+    return;
+  }
+  // lattice coordinates: crystallographic reciprocal lattice (no 2*pi)
+  std::vector<double> frac(3 * (size_t)natoms);
+  const double *A = latt_vec;
+  if (cartesian) {
+    double r[3][3], B[9];
+    auto vp = [](const double *a, const double *b, double *c) {
+      c[0] = a[1] * b[2] - a[2] * b[1];
+      c[1] = a[2] * b[0] - a[0] * b[2];
+      c[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    vp(A + 3, A + 6, r[0]);
+    vp(A + 0, A + 6, r[1]);
+    vp(A + 0, A + 3, r[2]);
+    for (int x = 0; x < 3; ++x) r[1][x] = -r[1][x];
+    auto a = [A](int i, int j) { return A[(j - 1) * 3 + (i - 1)]; };
+    double V = a(1, 1) * a(2, 2) * a(3, 3) + a(2, 1) * a(3, 2) * a(1, 3) + a(3, 1) * a(1, 2) * a(2, 3) -
+               a(3, 1) * a(2, 2) * a(1, 3) - a(1, 1) * a(3, 2) * a(2, 3) - a(2, 1) * a(1, 2) * a(3, 3);
+    for (int i = 0; i < 3; ++i)
+      for (int x = 0; x < 3; ++x) B[x * 3 + i] = r[i][x] / V;
+    for (int i = 0; i < natoms; ++i)
+      for (int rr = 0; rr < 3; ++rr) {
+        double acc = B[rr] * coords[3 * i] + B[3 + rr] * coords[3 * i + 1];
+        frac[3 * i + rr] = acc + B[6 + rr] * coords[3 * i + 2];
+      }
+  } else {
+    std::memcpy(frac.data(), coords, frac.size() * sizeof(double));
+  }
+  // the batch machinery takes Cartesian input; give it L*frac and a
+  // lattice whose reciprocal maps it back exactly is not guaranteed, so
+  // the lattice coordinates are handed over directly
+  Batch b;
+  std::string err;
+  int pbcv = pbc ? 1 : 0;
+  int st = b.create_lattice_coords(g.set, natoms, latt_vec, pbcv, err);
+  if (st) {
+    set_err(st, err);
+    return;
+  }
+  hipStream_t s = lib_stream();
+  DBuf<double> dfrac;
+  DBuf<int> dtypes;
+  if (dfrac.upload(frac.data(), frac.size(), s) || dtypes.upload(atom_types, natoms, s)) {
+    set_err(AENET_STATUS_ERR_MALLOC, "aenet_nbl_init: device allocation failed");
+    return;
+  }
+  int cap = std::min(g.set.nnb_max, 512);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    DBuf<int> cnt, lst, typ;
+    DBuf<double> coo, dst;
+    if (cnt.alloc(natoms) || lst.alloc((size_t)natoms * cap) || typ.alloc((size_t)natoms * cap) ||
+        coo.alloc(3 * (size_t)natoms * cap) || dst.alloc((size_t)natoms * cap)) {
+      set_err(AENET_STATUS_ERR_MALLOC, "aenet_nbl_init: device allocation failed");
+      return;
+    }
+    st = b.neighbor_lists(dfrac.p, dtypes.p, cap, cnt.p, coo.p, dst.p, lst.p, typ.p, s);
+    std::string ferr;
+    int fst = st ? st : b.check_flags(s, ferr);
+    if (fst == AENET_STATUS_ERR_CAPACITY && attempt == 0 && cap < g.set.nnb_max) {
+      cap = g.set.nnb_max;
+      continue;
+    }
+    if (fst) {
+      set_err(fst, ferr);
+      return;
+    }
+    g.nbl_count.resize(natoms);
+    g.nbl_list.resize((size_t)natoms * cap);
+    g.nbl_type.resize((size_t)natoms * cap);
+    g.nbl_coo.resize(3 * (size_t)natoms * cap);
+    g.nbl_dist.resize((size_t)natoms * cap);
+    (void)hipMemcpyAsync(g.nbl_count.data(), cnt.p, natoms * sizeof(int), hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(g.nbl_list.data(), lst.p, g.nbl_list.size() * sizeof(int), hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(g.nbl_type.data(), typ.p, g.nbl_type.size() * sizeof(int), hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(g.nbl_coo.data(), coo.p, g.nbl_coo.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+    (void)hipMemcpyAsync(g.nbl_dist.data(), dst.p, g.nbl_dist.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+    // wrapped coordinates back, Cartesian: matmul(latt_vec, coo_latt)
+    std::vector<double> wrapped(3 * (size_t)natoms);
+    (void)hipMemcpyAsync(wrapped.data(), b.wrapped_coords(), wrapped.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      set_err(AENET_STATUS_ERR_DEVICE, "aenet_nbl_init: device error");
+      return;
+    }
+    for (int i = 0; i < natoms; ++i)
+      for (int rr = 0; rr < 3; ++rr) {
+        double acc = A[rr] * wrapped[3 * i] + A[3 + rr] * wrapped[3 * i + 1];
+        coords[3 * i + rr] = acc + A[6 + rr] * wrapped[3 * i + 2];
+      }
+    g.nbl_natoms = natoms;
+    g.nbl_cap = cap;
+    g.nbl_init = true;
+    return;
+  }
+}
+
+void aenet_nbl_final(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  g.nbl_init = false;
+  g.nbl_count.clear();
+  g.nbl_list.clear();
+  g.nbl_type.clear();
+  g.nbl_coo.clear();
+  g.nbl_dist.clear();
+}
+
+void aenet_nbl_neighbors(int iatom, int *nnb, double nbcoo[], double nbdist[], int nblist[], int nbtype[]) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  if (!g.nbl_init || iatom < 1 || iatom > g.nbl_natoms) {
+    set_err(AENET_STATUS_ERR_INIT, "aenet_nbl_neighbors: no neighbor list for this atom");
+    *nnb = 0;
+    return;
+  }
+  int n = g.nbl_count[iatom - 1];
+  int m = std::min(n, *nnb);
+  size_t base = (size_t)(iatom - 1) * g.nbl_cap;
+  std::memcpy(nbcoo, &g.nbl_coo[3 * base], 3 * (size_t)m * sizeof(double));
+  std::memcpy(nbdist, &g.nbl_dist[base], (size_t)m * sizeof(double));
+  std::memcpy(nblist, &g.nbl_list[base], (size_t)m * sizeof(int));
+  std::memcpy(nbtype, &g.nbl_type[base], (size_t)m * sizeof(int));
+  *nnb = n;
+}
+
+// ---------------------------------------------------------------------
+// batched entry points (include/aenet_amd.h)
+// ---------------------------------------------------------------------
+
+void aenet_structures_energy_and_forces(int nstruct, const int natoms[], const double latt[], const int pbc[],
+                                        const int types[], const double coords[], double E_tot[],
+                                        double E_coh[], double E_atoms[], double F[], int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  if (!g.init || !g.set.loaded) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "potentials not loaded");
+    return;
+  }
+  std::string err;
+  Batch b;
+  int st = b.create(g.set, nstruct, natoms, latt, pbc, coords, err);
+  if (st) {
+    *stat = set_err(st, err);
+    return;
+  }
+  int N = b.natoms();
+  for (int i = 0; i < N; ++i)
+    if (types[i] < 1 || types[i] > g.set.ntypes) {
+      *stat = set_err(AENET_STATUS_ERR_TYPE, "atom type out of range");
+      return;
+    }
+  hipStream_t s = lib_stream();
+  DBuf<double> dc, dE, dES, dF;
+  DBuf<int> dt;
+  if (dc.upload(coords, 3 * (size_t)N, s) || dt.upload(types, N, s) || dE.alloc(N) || dES.alloc(2 * (size_t)nstruct) ||
+      (F && dF.alloc(3 * (size_t)N))) {
+    *stat = set_err(AENET_STATUS_ERR_MALLOC, "device allocation failed");
+    return;
+  }
+  st = b.eval(*g.dev, dc.p, dt.p, dE.p, dES.p, F ? dF.p : nullptr, s);
+  if (!st) st = b.check_flags(s, err);
+  if (st == AENET_STATUS_ERR_CAPACITY && b.nb_cap() < AENET_AMD_NB_CAP) {
+    // denser than estimated: once more with the largest list capacity
+    err.clear();
+    st = b.set_nb_cap(AENET_AMD_NB_CAP);
+    if (!st) st = b.eval(*g.dev, dc.p, dt.p, dE.p, dES.p, F ? dF.p : nullptr, s);
+    if (!st) st = b.check_flags(s, err);
+  }
+  if (st) {
+    *stat = set_err(st, err.empty() ? "kernel launch failed" : err);
+    return;
+  }
+  std::vector<double> es(2 * (size_t)nstruct);
+  (void)hipMemcpyAsync(es.data(), dES.p, es.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (E_atoms) (void)hipMemcpyAsync(E_atoms, dE.p, N * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (F) (void)hipMemcpyAsync(F, dF.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, s);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    *stat = set_err(AENET_STATUS_ERR_DEVICE, "device error");
+    return;
+  }
+  for (int i = 0; i < nstruct; ++i) {
+    if (E_tot) E_tot[i] = es[i];
+    if (E_coh) E_coh[i] = es[nstruct + i];
+  }
+  *stat = AENET_STATUS_OK;
+}
+
+void aenet_amd_batch_create(int nstruct, const int natoms[], const double latt[], const int pbc[],
+                            const double coords_host[], aenet_amd_batch **batch, int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  *batch = nullptr;
+  if (!g.init || !g.set.loaded) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "potentials not loaded");
+    return;
+  }
+  std::unique_ptr<aenet_amd_batch> b(new aenet_amd_batch());
+  std::string err;
+  int st = b->batch.create(g.set, nstruct, natoms, latt, pbc, coords_host, err);
+  if (st) {
+    *stat = set_err(st, err);
+    return;
+  }
+  *batch = b.release();
+  *stat = AENET_STATUS_OK;
+}
+
+void aenet_amd_batch_destroy(aenet_amd_batch *batch) { delete batch; }
+
+int aenet_amd_batch_natoms(const aenet_amd_batch *batch) { return batch ? batch->batch.natoms() : 0; }
+
+void aenet_amd_batch_eval(aenet_amd_batch *batch, const double *d_coords, const int *d_types, double *d_E_atoms,
+                          double *d_E_struct, double *d_F, void *stream, int *stat) {
+  Global &g = G();
+  if (!batch || !g.dev) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or potentials not loaded");
+    return;
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : lib_stream();
+  *stat = batch->batch.eval(*g.dev, d_coords, d_types, d_E_atoms, d_E_struct, d_F, s);
+  if (*stat) set_err(*stat, "kernel launch failed");
+}
+
+void aenet_amd_batch_set_timing(aenet_amd_batch *batch, int enable) {
+  if (batch) batch->batch.set_timing(enable != 0);
+}
+
+void aenet_amd_batch_kernel_time(aenet_amd_batch *batch, double *ms, int *count, int *stat) {
+  if (!batch) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch");
+    return;
+  }
+  *stat = batch->batch.kernel_time(*ms, *count);
+  if (*stat) set_err(*stat, "event timing failed");
+}
+
+void aenet_amd_batch_check(aenet_amd_batch *batch, void *stream, int *stat) {
+  if (!batch) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch");
+    return;
+  }
+  std::string err;
+  hipStream_t s = stream ? (hipStream_t)stream : lib_stream();
+  *stat = batch->batch.check_flags(s, err);
+  if (*stat) set_err(*stat, err);
+}
+
+void aenet_amd_batch_neighbors(aenet_amd_batch *batch, const double *d_coords, const int *d_types, int cap,
+                               int *d_count, double *d_coo, double *d_dist, int *d_list, int *d_type, void *stream,
+                               int *stat) {
+  if (!batch) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch");
+    return;
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : lib_stream();
+  *stat = batch->batch.neighbor_lists(d_coords, d_types, cap, d_count, d_coo, d_dist, d_list, d_type, s);
+  if (*stat) set_err(*stat, "kernel launch failed");
+}
+
+void aenet_amd_batch_descriptors(aenet_amd_batch *batch, const double *d_coords, const int *d_types, int scaled,
+                                 double *d_G, int stride, void *stream, int *stat) {
+  Global &g = G();
+  if (!batch || !g.dev) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or potentials not loaded");
+    return;
+  }
+  if (stride < g.set.nsf_max) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "descriptor stride smaller than nsf_max");
+    return;
+  }
+  hipStream_t s = stream ? (hipStream_t)stream : lib_stream();
+  *stat = batch->batch.descriptors(*g.dev, d_coords, d_types, scaled, d_G, stride, s);
+  if (*stat) set_err(*stat, "kernel launch failed");
+}
+
+}  // extern "C"

@@ -1,0 +1,381 @@
+// engine.cpp -- device-resident potential sets and structure batches.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "aenet_capi_status.h"
+#include "kernels.h"
+
+namespace aenet_amd {
+
+int DeviceModel::upload(const PotSet &ps, hipStream_t s) {
+  ntypes = ps.ntypes;
+  desc = ps.desc;
+  max_rN = ps.max_rN;
+  max_aN = ps.max_aN;
+  nsf_max = ps.nsf_max;
+  wsize = (int)ps.W.size();
+  nl_max = 0;
+  for (const SpeciesDev &sd : ps.species) nl_max = std::max(nl_max, sd.nlayers);
+  if (species.upload(ps.species.data(), ps.species.size(), s)) return AENET_STATUS_ERR_DEVICE;
+  if (W.upload(ps.W.data(), ps.W.size(), s)) return AENET_STATUS_ERR_DEVICE;
+  if (avg.upload(ps.norm_avg.data(), ps.norm_avg.size(), s)) return AENET_STATUS_ERR_DEVICE;
+  if (scale.upload(ps.norm_scale.data(), ps.norm_scale.size(), s)) return AENET_STATUS_ERR_DEVICE;
+  if (bp.upload(&ps.bp, 1, s)) return AENET_STATUS_ERR_DEVICE;
+  if (bpf.upload(ps.bpf.data(), ps.bpf.size(), s)) return AENET_STATUS_ERR_DEVICE;
+  if (hipStreamSynchronize(s) != hipSuccess) return AENET_STATUS_ERR_DEVICE;
+  return AENET_STATUS_OK;
+}
+
+ModelDev DeviceModel::view() const {
+  ModelDev m;
+  m.species = species.p;
+  m.W = W.p;
+  m.norm_avg = avg.p;
+  m.norm_scale = scale.p;
+  m.bp = bp.p;
+  m.bpf = bpf.p;
+  m.ntypes = ntypes;
+  m.desc = desc;
+  return m;
+}
+
+int Batch::create(const PotSet &ps, int nstruct, const int *natoms, const double *latt, const int *pbc,
+                  const double *coords, std::string &err) {
+  int st = build_plan(ps.Rc_max, nstruct, natoms, latt, pbc, coords, plan_, err);
+  if (st) return st;
+  return allocate(err);
+}
+
+int Batch::create_lattice_coords(const PotSet &ps, int natoms, const double *latt, int pbc, std::string &err) {
+  int st = build_plan_lattice(ps.Rc_max, natoms, latt, pbc, plan_, err);
+  if (st) return st;
+  return allocate(err);
+}
+
+int Batch::allocate(std::string &err) {
+  const PlanHost &p = plan_;
+  combo_cap_ = 1;
+  for (auto &d : p.st) combo_cap_ = std::max(combo_cap_, std::min((d.nC + 1) * (d.nT + 1), AENET_AMD_COMBO_CAP));
+  hipStream_t s = nullptr;
+  bool bad = false;
+  bad |= st_.upload(p.st.data(), p.st.size(), s) != 0;
+  bad |= atom_struct_.upload(p.atom_struct.data(), p.atom_struct.size(), s) != 0;
+  bad |= cell_struct_.upload(p.cell_struct.data(), p.cell_struct.size(), s) != 0;
+  bad |= cell_ijk_.upload(p.cell_ijk.data(), p.cell_ijk.size(), s) != 0;
+  bad |= Tvec_.upload(p.Tvec.data(), p.Tvec.size(), s) != 0;
+  bad |= Cvec_.upload(p.Cvec.data(), p.Cvec.size(), s) != 0;
+  bad |= self_t_.upload(p.self_t.data(), p.self_t.size(), s) != 0;
+  bad |= self_d_.upload(p.self_d.data(), p.self_d.size(), s) != 0;
+  bad |= cell_ctr_.upload(p.cell_ctr.data(), p.cell_ctr.size(), s) != 0;
+  bad |= s_.alloc(3 * (size_t)p.natoms) != 0;
+  bad |= sorted_s_.alloc(3 * (size_t)p.natoms) != 0;
+  bad |= cellof_.alloc(p.natoms) != 0;
+  bad |= cell_count_.alloc(p.ncells) != 0;
+  bad |= cell_fill_.alloc(p.ncells) != 0;
+  bad |= cell_start_.alloc(p.ncells + 1) != 0;
+  bad |= cell_atoms_.alloc(p.natoms) != 0;
+  bad |= cell_tmp_.alloc(p.natoms) != 0;
+  bad |= combo_count_.alloc(p.ncells) != 0;
+  bad |= combos_.alloc(2 * (size_t)p.ncells * combo_cap_) != 0;
+  bad |= flags_.alloc(1) != 0;
+  bad |= scratch_E_.alloc(p.natoms) != 0;
+  if (bad) {
+    err = "device allocation failed";
+    return AENET_STATUS_ERR_MALLOC;
+  }
+  // neighbor capacity from the densest structure: atoms within the list
+  // radius at that density, with headroom
+  double est = 0.0;
+  for (auto &d : p.st) {
+    double V = std::fabs(d.L[0] * (d.L[4] * d.L[8] - d.L[5] * d.L[7]) - d.L[3] * (d.L[1] * d.L[8] - d.L[2] * d.L[7]) +
+                         d.L[6] * (d.L[1] * d.L[5] - d.L[2] * d.L[4]));
+    double rho = V > 0.0 ? d.natoms / V : 0.0;
+    est = std::max(est, rho * 4.18879020478639 * d.rc_list * d.rc_list * d.rc_list + d.nself);
+  }
+  int cap = (int)(1.25 * est) + 32;
+  cap = std::max(64, std::min(AENET_AMD_NB_CAP, (cap + 31) / 32 * 32));
+  int st = set_nb_cap(cap);
+  if (st) {
+    err = "device allocation failed";
+    return st;
+  }
+  // hipMalloc does not clear memory: the error word and the cell counters
+  // must start at zero (each step leaves the counters zeroed again)
+  launch_fill_i32(flags_.p, 0, 1, s);
+  launch_fill_i32(cell_count_.p, 0, p.ncells, s);
+  launch_fill_i32(cell_fill_.p, 0, p.ncells, s);
+  // the (cell, image) combinations depend on the lattice only
+  launch_combos(view(nullptr, nullptr), combo_count_.p, combos_.p, s);
+  if (hipDeviceSynchronize() != hipSuccess) {
+    err = "device error";
+    return AENET_STATUS_ERR_DEVICE;
+  }
+  return check_flags(s, err);
+}
+
+int Batch::set_nb_cap(int cap) {
+  if (cap < 1) cap = 1;
+  if (cap > AENET_AMD_NB_CAP) cap = AENET_AMD_NB_CAP;
+  if (cap == nb_cap_ && nb_R_.p) return AENET_STATUS_OK;
+  size_t n = (size_t)plan_.natoms;
+  if (nb_count_.alloc(n) || nb_j_.alloc(n * cap) || nb_R_.alloc(4 * n * cap)) return AENET_STATUS_ERR_MALLOC;
+  nb_cap_ = cap;
+  return AENET_STATUS_OK;
+}
+
+NbDev Batch::nb_view() const {
+  NbDev nb;
+  nb.cap = nb_cap_;
+  nb.count = nb_count_.p;
+  nb.R = nb_R_.p;
+  nb.j = nb_j_.p;
+  return nb;
+}
+
+BatchDev Batch::view(const double *d_coords, const int *d_types) const {
+  BatchDev b;
+  std::memset(&b, 0, sizeof(b));
+  b.st = st_.p;
+  b.nstruct = (int)plan_.st.size();
+  b.natoms = plan_.natoms;
+  b.atom_struct = atom_struct_.p;
+  b.cart = d_coords;
+  b.types = d_types;
+  b.s = s_.p;
+  b.cellof = cellof_.p;
+  b.ncells = plan_.ncells;
+  b.cell_struct = cell_struct_.p;
+  b.cell_ijk = cell_ijk_.p;
+  b.cell_count = cell_count_.p;
+  b.cell_fill = cell_fill_.p;
+  b.cell_start = cell_start_.p;
+  b.cell_atoms = cell_atoms_.p;
+  b.cell_ctr = cell_ctr_.p;
+  b.sorted_s = sorted_s_.p;
+  b.combo_count = combo_count_.p;
+  b.combos = combos_.p;
+  b.combo_cap = combo_cap_;
+  b.Tvec = Tvec_.p;
+  b.Cvec = Cvec_.p;
+  b.self_t = self_t_.p;
+  b.self_d = self_d_.p;
+  b.flags = flags_.p;
+  return b;
+}
+
+int Batch::eval(const DeviceModel &dm, const double *d_coords, const int *d_types, double *d_E_atom,
+                double *d_E_struct, double *d_F, hipStream_t stream) {
+  BatchDev b = view(d_coords, d_types);
+  ModelDev md = dm.view();
+  NbDev nb = nb_view();
+  launch_binning(b, cell_tmp_.p, d_F, stream);
+  launch_search(b, nb, stream);
+  OutDev o;
+  std::memset(&o, 0, sizeof(o));
+  o.E_atom = d_E_atom ? d_E_atom : scratch_E_.p;
+  o.F = d_F;
+  EnvDev e;
+  std::memset(&e, 0, sizeof(e));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timing_ && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+    (void)hipEventRecord(e0, stream);
+  }
+  if (dm.desc == DESC_CHEBYSHEV)
+    launch_atom_cheb(dm.max_rN, dm.max_aN, 0, d_F != nullptr, dm.sizes(b.natoms, nb_cap_), md, b, nb, o, e,
+                     stream);
+  else
+    launch_atom_bp(0, d_F != nullptr, dm.sizes(b.natoms, nb_cap_), md, b, nb, o, e, stream);
+  if (e0 && e1) {
+    (void)hipEventRecord(e1, stream);
+    ev_.push_back(e0);
+    ev_.push_back(e1);
+  }
+  if (d_E_struct) {
+    // E_struct = [E_tot(0..ns-1), E_coh(0..ns-1)]
+    launch_struct_energy(b, md, o.E_atom, d_E_struct, d_E_struct + b.nstruct, stream);
+  }
+  return hipGetLastError() == hipSuccess ? AENET_STATUS_OK : AENET_STATUS_ERR_DEVICE;
+}
+
+int Batch::neighbor_lists(const double *d_coords, const int *d_types, int cap, int *d_count, double *d_coo,
+                          double *d_dist, int *d_list, int *d_type, hipStream_t stream) {
+  BatchDev b = view(d_coords, d_types);
+  launch_binning(b, cell_tmp_.p, nullptr, stream);
+  OutDev o;
+  std::memset(&o, 0, sizeof(o));
+  o.nb_cap = cap;
+  o.nb_count = d_count;
+  o.nb_coo = d_coo;
+  o.nb_dist = d_dist;
+  o.nb_list = d_list;
+  o.nb_type = d_type;
+  launch_neighbor_lists(b, o, stream);
+  return hipGetLastError() == hipSuccess ? AENET_STATUS_OK : AENET_STATUS_ERR_DEVICE;
+}
+
+int Batch::descriptors(const DeviceModel &dm, const double *d_coords, const int *d_types, int scaled,
+                       double *d_G, int stride, hipStream_t stream) {
+  BatchDev b = view(d_coords, d_types);
+  ModelDev md = dm.view();
+  NbDev nb = nb_view();
+  launch_binning(b, cell_tmp_.p, nullptr, stream);
+  launch_search(b, nb, stream);
+  OutDev o;
+  std::memset(&o, 0, sizeof(o));
+  o.E_atom = scratch_E_.p;
+  o.G = d_G;
+  o.G_scaled = scaled;
+  o.G_stride = stride;
+  o.desc_only = 1;
+  EnvDev e;
+  std::memset(&e, 0, sizeof(e));
+  if (dm.desc == DESC_CHEBYSHEV)
+    launch_atom_cheb(dm.max_rN, dm.max_aN, 0, false, dm.sizes(b.natoms, nb_cap_), md, b, nb, o, e, stream);
+  else
+    launch_atom_bp(0, false, dm.sizes(b.natoms, nb_cap_), md, b, nb, o, e, stream);
+  return hipGetLastError() == hipSuccess ? AENET_STATUS_OK : AENET_STATUS_ERR_DEVICE;
+}
+
+void Batch::set_timing(bool on) { timing_ = on; }
+
+int Batch::kernel_time(double &ms, int &count) {
+  ms = 0.0;
+  count = 0;
+  int st = AENET_STATUS_OK;
+  for (size_t i = 0; i + 1 < ev_.size(); i += 2) {
+    float t = 0.0f;
+    if (hipEventSynchronize(ev_[i + 1]) != hipSuccess || hipEventElapsedTime(&t, ev_[i], ev_[i + 1]) != hipSuccess)
+      st = AENET_STATUS_ERR_DEVICE;
+    ms += t;
+    ++count;
+  }
+  for (auto e : ev_) (void)hipEventDestroy(e);
+  ev_.clear();
+  return st;
+}
+
+Batch::~Batch() {
+  for (auto e : ev_) (void)hipEventDestroy(e);
+}
+
+static std::string flag_text(int f) {
+  std::string s;
+  if (f & AENET_AMD_FLAG_NB_OVERFLOW) s += " neighbor capacity exceeded;";
+  if (f & AENET_AMD_FLAG_COMBO_OVERFLOW) s += " cell-image combination capacity exceeded;";
+  if (f & AENET_AMD_FLAG_KEPT_OVERFLOW) s += " per-atom cell-image capacity exceeded;";
+  if (f & AENET_AMD_FLAG_BAD_TYPE) s += " atom type out of range;";
+  if (f & AENET_AMD_FLAG_TRIPLET_OVERFLOW) s += " triplet capacity exceeded;";
+  if (f & AENET_AMD_FLAG_BAD_COORD) s += " non-finite or out-of-range coordinates;";
+  return s;
+}
+
+int Batch::check_flags(hipStream_t stream, std::string &err) {
+  int f = 0;
+  if (hipMemcpyAsync(&f, flags_.p, sizeof(int), hipMemcpyDeviceToHost, stream) != hipSuccess ||
+      hipStreamSynchronize(stream) != hipSuccess) {
+    err = "device error";
+    return AENET_STATUS_ERR_DEVICE;
+  }
+  if (f) {
+    err = "kernel reported:" + flag_text(f);
+    int zero = 0;
+    (void)hipMemcpy(flags_.p, &zero, sizeof(int), hipMemcpyHostToDevice);
+    return (f & AENET_AMD_FLAG_BAD_TYPE) ? AENET_STATUS_ERR_TYPE : AENET_STATUS_ERR_CAPACITY;
+  }
+  return AENET_STATUS_OK;
+}
+
+int EnvScratch::eval(const DeviceModel &dm, int nenv, const int *h_start, const int *h_ctype, const double *h_ci,
+                     const double *h_cj, const int *h_tj, bool forces, double *h_E, double *h_dFi,
+                     double *h_dFj, double *h_G, int G_stride, hipStream_t stream, std::string &err) {
+  size_t nnb = (size_t)h_start[nenv];
+  int max_n = 1;
+  for (int i = 0; i < nenv; ++i) max_n = std::max(max_n, h_start[i + 1] - h_start[i]);
+  if (max_n > AENET_AMD_NB_CAP) {
+    err = "more than 512 neighbors in one environment";
+    return AENET_STATUS_ERR_CAPACITY;
+  }
+  max_n = (max_n + 31) / 32 * 32;
+  bool bad = false;
+  if ((size_t)nenv > cap_env || !start.p) {
+    cap_env = std::max((size_t)nenv, (size_t)64);
+    bad |= start.alloc(cap_env + 1) != 0;
+    bad |= ctype.alloc(cap_env) != 0;
+    bad |= ci.alloc(3 * cap_env) != 0;
+    bad |= E.alloc(cap_env) != 0;
+    bad |= dFi.alloc(3 * cap_env) != 0;
+    bad |= flags.alloc(1) != 0;
+  }
+  if (nnb > cap_nb || !cj.p) {
+    cap_nb = std::max(nnb, (size_t)1024);
+    bad |= cj.alloc(3 * cap_nb) != 0;
+    bad |= tj.alloc(cap_nb) != 0;
+    bad |= dFj.alloc(3 * cap_nb) != 0;
+  }
+  DBuf<double> G;
+  if (h_G) bad |= G.alloc((size_t)nenv * G_stride) != 0;
+  if (bad) {
+    err = "device allocation failed";
+    return AENET_STATUS_ERR_MALLOC;
+  }
+  (void)hipMemcpyAsync(start.p, h_start, (nenv + 1) * sizeof(int), hipMemcpyHostToDevice, stream);
+  (void)hipMemcpyAsync(ctype.p, h_ctype, nenv * sizeof(int), hipMemcpyHostToDevice, stream);
+  (void)hipMemcpyAsync(ci.p, h_ci, 3 * nenv * sizeof(double), hipMemcpyHostToDevice, stream);
+  if (nnb) {
+    (void)hipMemcpyAsync(cj.p, h_cj, 3 * nnb * sizeof(double), hipMemcpyHostToDevice, stream);
+    (void)hipMemcpyAsync(tj.p, h_tj, nnb * sizeof(int), hipMemcpyHostToDevice, stream);
+  }
+  launch_fill_i32(flags.p, 0, 1, stream);
+  BatchDev b;
+  std::memset(&b, 0, sizeof(b));
+  b.flags = flags.p;
+  OutDev o;
+  std::memset(&o, 0, sizeof(o));
+  o.G = h_G ? G.p : nullptr;
+  o.G_scaled = 0;
+  o.G_stride = G_stride;
+  o.desc_only = h_G && !h_E;
+  EnvDev e;
+  e.nenv = nenv;
+  e.env_start = start.p;
+  e.env_type = ctype.p;
+  e.coo_i = ci.p;
+  e.coo_j = cj.p;
+  e.type_j = tj.p;
+  e.E = E.p;
+  e.dF_i = dFi.p;
+  e.dF_j = dFj.p;
+  ModelDev md = dm.view();
+  NbDev nb;
+  std::memset(&nb, 0, sizeof(nb));
+  if (dm.desc == DESC_CHEBYSHEV)
+    launch_atom_cheb(dm.max_rN, dm.max_aN, 1, forces, dm.sizes(nenv, max_n), md, b, nb, o, e, stream);
+  else
+    launch_atom_bp(1, forces, dm.sizes(nenv, max_n), md, b, nb, o, e, stream);
+  if (hipGetLastError() != hipSuccess) {
+    err = "kernel launch failed";
+    return AENET_STATUS_ERR_DEVICE;
+  }
+  int f = 0;
+  (void)hipMemcpyAsync(&f, flags.p, sizeof(int), hipMemcpyDeviceToHost, stream);
+  if (h_E) (void)hipMemcpyAsync(h_E, E.p, nenv * sizeof(double), hipMemcpyDeviceToHost, stream);
+  if (forces) {
+    (void)hipMemcpyAsync(h_dFi, dFi.p, 3 * nenv * sizeof(double), hipMemcpyDeviceToHost, stream);
+    if (nnb) (void)hipMemcpyAsync(h_dFj, dFj.p, 3 * nnb * sizeof(double), hipMemcpyDeviceToHost, stream);
+  }
+  if (h_G) (void)hipMemcpyAsync(h_G, G.p, (size_t)nenv * G_stride * sizeof(double), hipMemcpyDeviceToHost, stream);
+  if (hipStreamSynchronize(stream) != hipSuccess) {
+    err = "device error";
+    return AENET_STATUS_ERR_DEVICE;
+  }
+  if (f) {
+    err = "kernel reported:" + flag_text(f);
+    return (f & AENET_AMD_FLAG_BAD_TYPE) ? AENET_STATUS_ERR_TYPE : AENET_STATUS_ERR_CAPACITY;
+  }
+  return AENET_STATUS_OK;
+}
+
+}  // namespace aenet_amd

@@ -1,0 +1,121 @@
+// engine.h -- device-resident potential sets and structure batches.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <string>
+#include <vector>
+
+#include "aenet_amd_internal.h"
+#include "model.h"
+#include "plan.h"
+
+namespace aenet_amd {
+
+// RAII device buffer
+template <class T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  int alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    if (hipMalloc((void **)&p, count * sizeof(T)) != hipSuccess) {
+      p = nullptr;
+      return -1;
+    }
+    n = count;
+    return 0;
+  }
+  int upload(const T *src, size_t count, hipStream_t s) {
+    if (alloc(count)) return -1;
+    if (count && hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
+    return 0;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct DeviceModel {
+  DBuf<SpeciesDev> species;
+  DBuf<float> W;
+  DBuf<double> avg, scale;
+  DBuf<BPDev> bp;
+  DBuf<BPFunc> bpf;
+  int ntypes = 0, desc = 0, max_rN = 0, max_aN = 0, nsf_max = 0, wsize = 0, nl_max = 0;
+  AtomSizes sizes(int natoms, int cap) const { return AtomSizes{natoms, cap, nsf_max, wsize, nl_max}; }
+  int upload(const PotSet &ps, hipStream_t s);
+  ModelDev view() const;
+};
+
+// a fixed set of structures (shapes, lattices, atom counts); coordinates
+// and types may change between evaluations
+class Batch {
+ public:
+  int create(const PotSet &ps, int nstruct, const int *natoms, const double *latt, const int *pbc,
+             const double *coords, std::string &err);
+  // one structure given in lattice coordinates of latt (aenet_nbl_init)
+  int create_lattice_coords(const PotSet &ps, int natoms, const double *latt, int pbc, std::string &err);
+  const double *wrapped_coords() const { return s_.p; }
+  // all pointers are device pointers; E_struct receives 2 per structure
+  // (total, cohesive); F may be null when forces are not wanted
+  int eval(const DeviceModel &dm, const double *d_coords, const int *d_types, double *d_E_atom,
+           double *d_E_struct, double *d_F, hipStream_t stream);
+  // bit-exact neighbor lists of every atom
+  int neighbor_lists(const double *d_coords, const int *d_types, int cap, int *d_count, double *d_coo,
+                     double *d_dist, int *d_list, int *d_type, hipStream_t stream);
+  // descriptor values of every atom (FP64, stride nsf_max)
+  int descriptors(const DeviceModel &dm, const double *d_coords, const int *d_types, int scaled,
+                  double *d_G, int stride, hipStream_t stream);
+  int check_flags(hipStream_t stream, std::string &err);
+  // HIP events around the atom kernel of every evaluation while enabled
+  void set_timing(bool on);
+  // summed duration (ms) and count of the timed atom-kernel launches since
+  // the last call; synchronizes on the recorded events
+  int kernel_time(double &ms, int &count);
+  ~Batch();
+  int natoms() const { return plan_.natoms; }
+  int nb_cap() const { return nb_cap_; }
+  int set_nb_cap(int cap);
+  int nstruct() const { return (int)plan_.st.size(); }
+  const PlanHost &plan() const { return plan_; }
+
+ private:
+  BatchDev view(const double *d_coords, const int *d_types) const;
+  int allocate(std::string &err);
+  PlanHost plan_;
+  DBuf<StructDev> st_;
+  DBuf<int> atom_struct_, cell_struct_, cell_ijk_, Tvec_, Cvec_, self_t_;
+  DBuf<double> self_d_;
+  DBuf<double> s_;
+  DBuf<int> cellof_, cell_count_, cell_fill_, cell_start_, cell_atoms_, cell_tmp_, combo_count_, combos_, flags_;
+  DBuf<double> cell_ctr_, sorted_s_;
+  int combo_cap_ = 0;
+  // compact neighbor lists
+  int nb_cap_ = 0;
+  DBuf<int> nb_count_, nb_j_;
+  DBuf<float> nb_R_;
+  NbDev nb_view() const;
+  bool timing_ = false;
+  std::vector<hipEvent_t> ev_;
+  DBuf<double> scratch_E_;
+};
+
+// explicit environments (per-atom entry points)
+struct EnvScratch {
+  DBuf<int> start, ctype, tj, flags;
+  DBuf<double> ci, cj, E, dFi, dFj;
+  size_t cap_env = 0, cap_nb = 0;
+  int eval(const DeviceModel &dm, int nenv, const int *h_start, const int *h_ctype, const double *h_ci,
+           const double *h_cj, const int *h_tj, bool forces, double *h_E, double *h_dFi, double *h_dFj,
+           double *h_G, int G_stride, hipStream_t stream, std::string &err);
+};
+
+}  // namespace aenet_amd

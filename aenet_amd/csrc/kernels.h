@@ -1,0 +1,45 @@
+// kernels.h -- host-side launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "aenet_amd_internal.h"
+
+namespace aenet_amd {
+
+// kernels.hip
+void launch_fill_i32(int *p, int v, long n, hipStream_t s);
+void launch_fill_f64(double *p, double v, long n, hipStream_t s);
+// (cell, image) combinations of every home cell; once per batch
+void launch_combos(const BatchDev &b, int *combo_count, int *combos, hipStream_t s);
+// cell binning of a batch (cell_count/cell_fill are cleared here); tmp: natoms ints
+// F (3 per atom, may be null) is cleared on the way
+void launch_binning(const BatchDev &b, int *tmp, double *F, hipStream_t s);
+// compact neighbor lists for the atom kernels
+void launch_search(const BatchDev &b, const NbDev &nb, hipStream_t s);
+// bit-exact FP64 neighbor lists
+void launch_neighbor_lists(const BatchDev &b, const OutDev &o, hipStream_t s);
+void launch_struct_energy(const BatchDev &b, const ModelDev &md, const double *E_atom, double *E_tot,
+                          double *E_coh, hipStream_t s);
+
+// atom_cheb.hip: mode 0 = structures, 1 = explicit environments
+// properties of the current device the launchers size their grids by
+struct DeviceLimits {
+  int cus;               // compute units
+  size_t lds_per_block;  // bytes of LDS one workgroup may allocate
+  size_t lds_per_cu;     // bytes of LDS per compute unit
+};
+const DeviceLimits &device_limits();
+
+// set bits of the batch's error word from the host side of a launch
+void raise_flag(int *flags, int bits, hipStream_t s);
+
+void launch_atom_cheb(int rN, int aN, int mode, bool forces, const AtomSizes &z, const ModelDev &md,
+                      const BatchDev &b, const NbDev &nb, const OutDev &o, const EnvDev &e,
+                      hipStream_t stream);
+
+// atom_bp.hip
+void launch_atom_bp(int mode, bool forces, const AtomSizes &z, const ModelDev &md, const BatchDev &b,
+                    const NbDev &nb, const OutDev &o, const EnvDev &e, hipStream_t stream);
+
+}  // namespace aenet_amd

@@ -1,0 +1,238 @@
+// kernels_common.h -- wavefront helpers and the ordered FP64 neighbor
+// search shared by the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "aenet_amd_internal.h"
+
+#define WAVE 64
+
+namespace aenet_amd {
+
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+__device__ __forceinline__ unsigned long long wave_ballot(bool p) { return __ballot(p); }
+
+// number of set bits of mask below this lane
+__device__ __forceinline__ int mask_prefix(unsigned long long mask) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+__device__ __forceinline__ int popc64(unsigned long long m) { return __popcll(m); }
+
+template <class T>
+__device__ __forceinline__ T uniform(T v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+
+// inclusive prefix sum over the wavefront
+__device__ __forceinline__ int wave_scan(int x) {
+  for (int o = 1; o < WAVE; o <<= 1) {
+    int y = __shfl_up(x, o, WAVE);
+    if (lane_id() >= o) x += y;
+  }
+  return x;
+}
+
+// L*v for a column-major 3x3 lattice, summed in the reference's order
+// (matmul of src/ext/lclist.f90:621-651) without contraction.
+__device__ __forceinline__ void matvec_exact(const double *L, const double *v, double *out) {
+#pragma clang fp contract(off)
+  for (int r = 0; r < 3; ++r) {
+    double acc = L[r] * v[0] + L[3 + r] * v[1];
+    out[r] = acc + L[6 + r] * v[2];
+  }
+}
+
+__device__ __forceinline__ double dist2_exact(const double *c) {
+#pragma clang fp contract(off)
+  double d2 = c[0] * c[0] + c[1] * c[1];
+  return d2 + c[2] * c[2];
+}
+
+__device__ __forceinline__ void add_T_exact(const double *s, const int *T, double *out) {
+#pragma clang fp contract(off)
+  out[0] = s[0] + (double)T[0];
+  out[1] = s[1] + (double)T[1];
+  out[2] = s[2] + (double)T[2];
+}
+
+// LDS work space of one search
+struct SearchLds {
+  int kept[2 * AENET_AMD_KEPT_CAP];   // (cell, image rank) kept for this atom
+  int gk[AENET_AMD_KEPT_CAP];         // group: first kept entry
+  int gn[AENET_AMD_KEPT_CAP];         // group: number of images
+  int gpos[AENET_AMD_KEPT_CAP];       // group: first atom position in cell order
+  int gitems[AENET_AMD_KEPT_CAP + 1]; // group: prefix of (atoms x images)
+};
+
+// Ordered neighbor search of atom g (global index) for one wavefront.  The
+// entries come out in the order lcl_nbdist_cart produces them
+// (src/ext/lclist.f90:520-677): periodic self images first, then the
+// candidates in linked-cell order (home cell, then the stencil cells; atoms
+// in descending index within a cell), each candidate with its home image
+// followed by the lattice translations in the reference's order.  Only the
+// (cell, image) combinations of the home cell whose bounding sphere reaches
+// this atom's cutoff sphere are visited; the test that decides membership
+// is the reference's FP64 expression, so the list is the reference's.
+//
+// Sink::put(slot, j, coo2, cart, dist) receives every entry and must check
+// slot against its own capacity.  Returns the number of entries.
+template <class Sink>
+__device__ int search_neighbors(const BatchDev &b, int g, Sink &sink, SearchLds &W) {
+  const int lane = lane_id();
+  const int sid = uniform(b.atom_struct[g]);
+  const StructDev *S = b.st + sid;
+  double L[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) L[k] = S->L[k];
+  const double si[3] = {b.s[3 * (size_t)g], b.s[3 * (size_t)g + 1], b.s[3 * (size_t)g + 2]};
+  double coo1[3];
+  matvec_exact(L, si, coo1);
+  const double Rc2 = S->Rc2;
+  const int *Tv = b.Tvec + 3 * S->T0;
+  int count = 0;
+  // (2) periodic images of the central atom
+  const int nself = S->nself;
+  for (int base = 0; base < nself; base += WAVE) {
+    int q = base + lane;
+    if (q < nself) {
+      int t = b.self_t[S->self0 + q];
+      double c2[3], coo2[3], cart[3];
+      add_T_exact(si, Tv + 3 * (t - 1), c2);
+      matvec_exact(L, c2, coo2);
+      cart[0] = coo2[0] - coo1[0];
+      cart[1] = coo2[1] - coo1[1];
+      cart[2] = coo2[2] - coo1[2];
+      sink.put(count + q, g, coo2, cart, b.self_d[S->self0 + q]);
+    }
+    count += min(WAVE, nself - base);
+  }
+  // (3) candidates: keep the home cell's (cell, image) combinations that
+  // can reach this atom
+  const int h = b.cellof[g];
+  const int K = b.combo_count[h];
+  const int *combos = b.combos + 2 * (size_t)h * b.combo_cap;
+  const double lim = S->rc_list + S->cell_rad + 1.0e-6;
+  const double lim2 = lim * lim;
+  int nk = 0;
+  for (int base = 0; base < K; base += WAVE) {
+    int q = base + lane;
+    bool keep = false;
+    int c = 0, t = 0;
+    if (q < K) {
+      c = combos[2 * q];
+      t = combos[2 * q + 1];
+      double ctr[3] = {b.cell_ctr[3 * c], b.cell_ctr[3 * c + 1], b.cell_ctr[3 * c + 2]};
+      if (t > 0) {
+        const int *T = Tv + 3 * (t - 1);
+        for (int r = 0; r < 3; ++r) ctr[r] += L[r] * T[0] + L[3 + r] * T[1] + L[6 + r] * T[2];
+      }
+      double dx = ctr[0] - coo1[0], dy = ctr[1] - coo1[1], dz = ctr[2] - coo1[2];
+      keep = (dx * dx + dy * dy + dz * dz <= lim2) && (b.cell_start[c + 1] > b.cell_start[c]);
+    }
+    unsigned long long m = wave_ballot(keep);
+    int pos = nk + mask_prefix(m);
+    if (keep && pos < AENET_AMD_KEPT_CAP) {
+      W.kept[2 * pos] = c;
+      W.kept[2 * pos + 1] = t;
+    }
+    nk += popc64(m);
+  }
+  if (nk > AENET_AMD_KEPT_CAP) {
+    if (lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_KEPT_OVERFLOW);
+    nk = AENET_AMD_KEPT_CAP;
+  }
+  __syncthreads();
+  // groups = runs of equal cell; items = atoms x images of each group
+  int ng = 0;
+  for (int base = 0; base < nk; base += WAVE) {
+    int q = base + lane;
+    bool start = q < nk && (q == 0 || W.kept[2 * q] != W.kept[2 * (q - 1)]);
+    unsigned long long m = wave_ballot(start);
+    if (start) {
+      int k = ng + mask_prefix(m);
+      int c = W.kept[2 * q];
+      W.gk[k] = q;
+      W.gpos[k] = b.cell_start[c];
+    }
+    ng += popc64(m);
+  }
+  __syncthreads();
+  int carry = 0;
+  for (int base = 0; base < ng; base += WAVE) {
+    int k = base + lane;
+    int items = 0;
+    if (k < ng) {
+      int q0 = W.gk[k];
+      int q1 = k + 1 < ng ? W.gk[k + 1] : nk;
+      int c = W.kept[2 * q0];
+      W.gn[k] = q1 - q0;
+      items = (b.cell_start[c + 1] - b.cell_start[c]) * (q1 - q0);
+    }
+    int incl = wave_scan(items);
+    if (k < ng) W.gitems[k] = carry + incl - items;
+    carry += __shfl(incl, WAVE - 1, WAVE);
+  }
+  if (lane == 0) W.gitems[ng] = carry;
+  __syncthreads();
+  const int total = carry;
+  for (int base = 0; base < total; base += WAVE) {
+    int q = base + lane;
+    bool pass = false;
+    int j = 0;
+    double coo2[3], cart[3], d2 = 0.0;
+    if (q < total) {
+      // group of item q: last k with gitems[k] <= q
+      int lo = 0, hi = ng - 1;
+      while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (W.gitems[mid] <= q)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      int off = q - W.gitems[lo];
+      int gn = W.gn[lo];
+      int a = off / gn;
+      int tt = off - a * gn;
+      int pos = W.gpos[lo] + a;
+      j = b.cell_atoms[pos];
+      int t = W.kept[2 * (W.gk[lo] + tt) + 1];
+      if (j != g) {
+        const double sj[3] = {b.sorted_s[3 * (size_t)pos], b.sorted_s[3 * (size_t)pos + 1],
+                              b.sorted_s[3 * (size_t)pos + 2]};
+        if (t == 0) {
+          matvec_exact(L, sj, coo2);
+        } else {
+          double c2[3];
+          add_T_exact(sj, Tv + 3 * (t - 1), c2);
+          matvec_exact(L, c2, coo2);
+        }
+        cart[0] = coo2[0] - coo1[0];
+        cart[1] = coo2[1] - coo1[1];
+        cart[2] = coo2[2] - coo1[2];
+        d2 = dist2_exact(cart);
+        pass = d2 <= Rc2;
+      }
+    }
+    unsigned long long msk = wave_ballot(pass);
+    if (pass) sink.put(count + mask_prefix(msk), j, coo2, cart, sqrt(d2));
+    count += popc64(msk);
+  }
+  return count;
+}
+
+}  // namespace aenet_amd

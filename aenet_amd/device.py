@@ -1,0 +1,118 @@
+"""Device-resident structure batches (torch tensors in HBM).
+
+Wraps the aenet_amd_batch_* entry points: the structures' shapes are fixed
+at creation; coordinates and types are device tensors handed to each
+evaluation.  torch provides the device memory and the stream; all compute
+is in libaenet_amd.so.
+"""
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _native
+from .core import check_status
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class DeviceBatch:
+    """A fixed set of structures evaluated on the GPU.
+
+    structures: list of objects with .cell (3x3, rows = vectors), .pbc,
+    .positions (N,3) and .types (1-based ids of the loaded potentials)."""
+
+    def __init__(self, structures, device="cuda"):
+        self.device = torch.device(device)
+        self.nstruct = len(structures)
+        self.natoms_per = np.array([len(s.types) for s in structures], dtype=np.int32)
+        latt = np.ascontiguousarray(np.concatenate([np.asarray(s.cell, np.float64).reshape(9) for s in structures]))
+        pbc = np.array([1 if s.pbc else 0 for s in structures], dtype=np.int32)
+        coords = np.ascontiguousarray(np.concatenate([np.asarray(s.positions, np.float64) for s in structures]))
+        types = np.ascontiguousarray(np.concatenate([np.asarray(s.types, np.int32) for s in structures]))
+        h = C.c_void_p()
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_create(self.nstruct, self.natoms_per, latt, pbc, coords.reshape(-1),
+                                             C.byref(h), C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_create")
+        self._h = h
+        self.natoms = int(self.natoms_per.sum())
+        self.offsets = np.concatenate([[0], np.cumsum(self.natoms_per)])
+        self.coords = torch.tensor(coords, dtype=torch.float64, device=self.device)
+        self.types = torch.tensor(types, dtype=torch.int32, device=self.device)
+        self.E_atoms = torch.zeros(self.natoms, dtype=torch.float64, device=self.device)
+        self.E_struct = torch.zeros(2 * self.nstruct, dtype=torch.float64, device=self.device)
+        self.forces = torch.zeros((self.natoms, 3), dtype=torch.float64, device=self.device)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            _native.lib().aenet_amd_batch_destroy(h)
+            self._h = None
+
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def eval(self, coords=None, types=None, forces=True, check=True):
+        """Launch one evaluation on the current torch stream; returns
+        (E_tot, E_coh, E_atoms, forces) device tensors."""
+        coords = self.coords if coords is None else coords
+        types = self.types if types is None else types
+        assert coords.is_contiguous() and coords.dtype == torch.float64 and coords.numel() == 3 * self.natoms
+        assert types.is_contiguous() and types.dtype == torch.int32 and types.numel() == self.natoms
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_eval(self._h, _ptr(coords), _ptr(types), _ptr(self.E_atoms),
+                                           _ptr(self.E_struct), _ptr(self.forces) if forces else None,
+                                           self._stream(), C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_eval")
+        if check:
+            self.check()
+        return (self.E_struct[:self.nstruct], self.E_struct[self.nstruct:], self.E_atoms,
+                self.forces if forces else None)
+
+    def check(self):
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_check(self._h, self._stream(), C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_check")
+
+    def set_timing(self, on):
+        _native.lib().aenet_amd_batch_set_timing(self._h, 1 if on else 0)
+
+    def kernel_time(self):
+        """(summed ms, launches) of the atom kernel since the last call."""
+        ms = C.c_double(0.0)
+        n = C.c_int(0)
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_kernel_time(self._h, C.byref(ms), C.byref(n), C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_kernel_time")
+        return ms.value, n.value
+
+    def neighbors(self, cap, coords=None, types=None):
+        coords = self.coords if coords is None else coords
+        types = self.types if types is None else types
+        d = self.device
+        n = self.natoms
+        count = torch.zeros(n, dtype=torch.int32, device=d)
+        coo = torch.zeros((n, cap, 3), dtype=torch.float64, device=d)
+        dist = torch.zeros((n, cap), dtype=torch.float64, device=d)
+        lst = torch.zeros((n, cap), dtype=torch.int32, device=d)
+        typ = torch.zeros((n, cap), dtype=torch.int32, device=d)
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_neighbors(self._h, _ptr(coords), _ptr(types), cap, _ptr(count), _ptr(coo),
+                                                _ptr(dist), _ptr(lst), _ptr(typ), self._stream(), C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_neighbors")
+        self.check()
+        return {"nnb": count, "coo": coo, "dist": dist, "list": lst, "type": typ}
+
+    def descriptors(self, stride, scaled=False, coords=None, types=None):
+        coords = self.coords if coords is None else coords
+        types = self.types if types is None else types
+        G = torch.zeros((self.natoms, stride), dtype=torch.float64, device=self.device)
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_descriptors(self._h, _ptr(coords), _ptr(types), int(scaled), _ptr(G), stride,
+                                                  self._stream(), C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_descriptors")
+        self.check()
+        return G

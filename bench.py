@@ -1,0 +1,239 @@
+#!/usr/bin/env python3
+"""Energy+force throughput of the 4000-atom binary alloy on one MI355X.
+
+One step = one full pass of the prediction path over the structure held in
+HBM: Cartesian -> lattice coordinates, cell binning, ordered FP64 neighbor
+search, Chebyshev descriptors (radial 18 / 6.5 A, angular 6 / 4.5 A,
+typespin), per-species 52-30-30-1 network forward and backward, forces
+(FP64 accumulation) and per-structure energy sums.
+
+Prints one JSON line.  ``roofline`` is for the fused atom kernel, timed with
+HIP events on its own stream; ``cpu_baseline`` is the reference library
+(built from its sources, oracle/_ref) run the way predict.x runs it, in
+worker processes over the host cores, on a bounded sample of atoms.
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom energy+force evals/s (1 GPU, FP32 vs FP64 ref)"
+FP32_PEAK = 157.3e12      # MI355X dense FP32 vector, FLOP/s
+HBM_PEAK = 8.0e12         # B/s
+# algorithmic FP32 FLOPs (FMA = 2) per unit of the Chebyshev path with
+# N_r = 18, N_a = 6 and typespin (m = 2): per pair 24 + (N_r+1)(12+4(m-1)),
+# per triplet 61 + (N_a+1)(11+4(m-1)), network 4*Wsize per atom, 3*nsf
+# for the normalization
+FLOP_PAIR = 24 + 19 * 16
+FLOP_TRIPLET = 61 + 7 * 15
+WSIZE = 53 * 30 + 31 * 30 + 31
+FLOP_ATOM = 4 * WSIZE + 3 * 52
+
+
+def pot_files():
+    d = os.path.join(ROOT, "tests", "golden", "potentials")
+    return {"A": os.path.join(d, "alloy.A.nn"), "B": os.path.join(d, "alloy.B.nn")}
+
+
+# ---------------------------------------------------------------------------
+# CPU baseline: the reference library, one process per core
+# ---------------------------------------------------------------------------
+
+def _cpu_worker(args):
+    rank, nranks = args
+    import ctypes as C
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Reference
+    from aenet_amd import synth
+    s = synth.c2_alloy4000()
+    ref = Reference(["A", "B"], pot_files())
+    lib = ref.lib()
+    lib.refdrv_timed.argtypes = [np.ctypeslib.ndpointer(np.float64), C.c_int, C.c_int,
+                                 np.ctypeslib.ndpointer(np.int32), np.ctypeslib.ndpointer(np.float64), C.c_int,
+                                 C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_int)]
+    n = len(s)
+    frac = np.zeros((n, 3))
+    L = np.zeros(9)
+    lib.refdrv_cart_to_frac(np.ascontiguousarray(s.cell).ravel(), 1, n, np.ascontiguousarray(s.positions).ravel(),
+                            frac, L)
+    tn, te, et = C.c_double(), C.c_double(), C.c_double()
+    st = C.c_int()
+    lib.refdrv_timed(L, 1, n, np.ascontiguousarray(s.types, np.int32), frac, rank, nranks, C.byref(tn), C.byref(te),
+                     C.byref(et), C.byref(st))
+    count = len(range(rank, n, nranks))
+    return count, tn.value, te.value
+
+
+def cpu_baseline(procs, stride):
+    """Reference library on `procs` worker processes; every worker takes
+    the atoms i = rank (mod procs*stride), so procs*stride sets the sample."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_lib import Reference
+    if not Reference.available():
+        raise RuntimeError("oracle/_ref/libaenet_ref.so not built (make -C oracle ref)")
+    ctx = mp.get_context("fork")
+    nranks = procs * stride
+    t0 = time.time()
+    with ctx.Pool(procs) as pool:
+        res = pool.map(_cpu_worker, [(r, nranks) for r in range(procs)])
+    wall = time.time() - t0
+    atoms = sum(r[0] for r in res)
+    t_eval = max(r[2] for r in res)
+    t_e2e = max(r[1] + r[2] for r in res)
+    cpu_s = sum(r[1] + r[2] for r in res)
+    return {
+        "value": atoms / t_eval,
+        "unit": "atoms/s",
+        "cores": procs,
+        "kind": "reference",
+        "sample": "%d of the 4000 atoms (every %dth), %d worker processes over disjoint atoms; "
+                  "value = aenet_atomic_energy_and_forces on precomputed lclist neighbor lists "
+                  "(kernel-equivalent); end_to_end includes lcl_init + lcl_nbdist_cart as predict.x runs them"
+                  % (atoms, stride, procs),
+        "end_to_end": atoms / t_e2e,
+        "cpu_seconds": cpu_s,
+        "wall_seconds": wall,
+    }
+
+
+def sys_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+# ---------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------
+
+def pair_triplet_counts(db, rc_r=6.5, rc_a=4.5):
+    import torch
+    nb = db.neighbors(256)
+    n = nb["nnb"].long()
+    d = nb["dist"]
+    idx = torch.arange(d.shape[1], device=d.device)[None, :]
+    valid = idx < n[:, None]
+    n_p = ((d <= rc_r) & (d > 1e-12) & valid).sum(dim=1)
+    m_a = ((d <= rc_a) & (d >= 1e-12) & valid).sum(dim=1)
+    n_t = m_a * (m_a - 1) // 2
+    return int(n_p.sum().item()), int(n_t.sum().item())
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--replicas", type=int, default=1,
+                    help="independent 4000-atom cells per step (default 1: the single alloy cell)")
+    ap.add_argument("--cpu-procs", type=int, default=0, help="CPU baseline worker processes (0: host cores, <=16)")
+    ap.add_argument("--cpu-stride", type=int, default=1, help="sample = every (procs*stride)th atom")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the prediction path runs on a single MI355X and this build "
+              "has no multi-GPU mode (--gpus %d requested)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        procs = args.cpu_procs or min(16, sys_cores())
+        try:
+            cpu = cpu_baseline(procs, args.cpu_stride)
+        except Exception as e:  # reference not built: report, do not fake
+            cpu = {"value": None, "unit": "atoms/s", "cores": procs, "kind": "reference",
+                   "sample": "unavailable: %s" % e}
+
+    import torch
+    from aenet_amd import synth
+    from aenet_amd.core import ANNPotentials
+    from aenet_amd.device import DeviceBatch
+
+    torch.cuda.set_device(0)
+    ann = ANNPotentials(pot_files())
+    if args.replicas == 1:
+        structs = [synth.c2_alloy4000()]
+    else:
+        structs = synth.alloy_replicas(args.replicas)
+    db = DeviceBatch(structs)
+    natoms = db.natoms
+    for _ in range(args.warmup):
+        db.eval(check=False)
+    db.check()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        db.eval(check=False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    db.check()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = natoms / (ms * 1e-3)
+    E_tot = float(db.E_struct[0].item())
+
+    # roofline of the fused atom kernel: HIP events around its launches
+    db.set_timing(True)
+    nt = min(args.steps, 100)
+    for _ in range(nt):
+        db.eval(check=False)
+    kms, kcount = db.kernel_time()
+    db.set_timing(False)
+    k_avg_s = kms / kcount * 1e-3
+    n_p, n_t = pair_triplet_counts(db)
+    flops = n_p * FLOP_PAIR + n_t * FLOP_TRIPLET + natoms * FLOP_ATOM
+    achieved = flops / k_avg_s
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atoms/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "alloy4000: 4000-atom binary A/B FCC cell (10x10x10, a=3.8 A, sigma=0.08 A), "
+                        "Chebyshev radial 18/6.5 A + angular 6/4.5 A with typespin (52 features), "
+                        "52-30-30-1 tanh network per species",
+            "structures_per_step": len(structs),
+            "atoms_per_step": natoms,
+            "E_tot_eV": E_tot,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "pipe": "FP32 VALU (MI355X FP32 vector and FP32 matrix peaks are both 157.3 TFLOP/s)",
+            "kernel": "k_atom_cheb<19,7>",
+            "achieved": achieved / 1e12,
+            "peak": FP32_PEAK / 1e12,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK,
+            "traffic": None,
+            "kernel_ms": k_avg_s * 1e3,
+            "flop_per_launch": flops,
+            "pairs": n_p,
+            "triplets": n_t,
+        },
+        "cpu_baseline": cpu,
+    }
+    if cpu and cpu.get("value"):
+        out["gpu_over_cpu"] = value / cpu["value"]
+        out["gpu_over_cpu_end_to_end"] = value / cpu["end_to_end"]
+    ann.release()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -149,4 +149,71 @@ contains
 
   end subroutine refdrv_energy
 
+  ! Timed evaluation of the atoms with mod(i-1,nranks)==rank, for the CPU
+  ! baseline: t_nbl = lcl_init plus the neighbor lists of those atoms
+  ! (lcl_nbdist_cart), t_eval = aenet_atomic_energy_and_forces on the
+  ! stored lists (the "kernel-equivalent" part); wall-clock seconds.
+  subroutine refdrv_timed(latt, pbc, natoms, types, frac, rank, nranks, &
+                          t_nbl, t_eval, Etot, stat) bind(C, name='refdrv_timed')
+    real(c_double), dimension(3,3),              intent(in)    :: latt
+    integer(c_int), value,                       intent(in)    :: pbc
+    integer(c_int), value,                       intent(in)    :: natoms
+    integer(c_int), dimension(natoms),   target, intent(in)    :: types
+    real(c_double), dimension(3,natoms), target, intent(inout) :: frac
+    integer(c_int), value,                       intent(in)    :: rank
+    integer(c_int), value,                       intent(in)    :: nranks
+    real(c_double),                              intent(out)   :: t_nbl
+    real(c_double),                              intent(out)   :: t_eval
+    real(c_double),                              intent(out)   :: Etot
+    integer(c_int),                              intent(out)   :: stat
+
+    integer,          dimension(:,:),   allocatable :: nblist, nbtype
+    double precision, dimension(:,:,:), allocatable :: nbcoo
+    double precision, dimension(:,:),   allocatable :: nbdist, cooi
+    integer,          dimension(:),     allocatable :: nnb, idx
+    double precision, dimension(:,:),   allocatable :: F
+    double precision :: E_i
+    integer :: iatom, k, nsub, st
+    integer(8) :: c0, c1, crate
+    logical :: pbc_f
+
+    pbc_f = (pbc /= 0)
+    stat = 0
+    nsub = 0
+    do iatom = 1, natoms
+       if (mod(iatom-1, nranks) == rank) nsub = nsub + 1
+    end do
+    allocate(nblist(aenet_nnb_max,nsub), nbtype(aenet_nnb_max,nsub), &
+             nbcoo(3,aenet_nnb_max,nsub), nbdist(aenet_nnb_max,nsub), &
+             cooi(3,nsub), nnb(nsub), idx(nsub), F(3,natoms))
+    call system_clock(c0, crate)
+    call lcl_init(aenet_Rc_min, aenet_Rc_max, latt, natoms, types, frac, pbc_f)
+    k = 0
+    do iatom = 1, natoms
+       if (mod(iatom-1, nranks) /= rank) cycle
+       k = k + 1
+       idx(k) = iatom
+       cooi(1:3,k) = matmul(latt, frac(1:3,iatom))
+       nnb(k) = aenet_nnb_max
+       call lcl_nbdist_cart(iatom, nnb(k), nbcoo(:,:,k), nbdist(:,k), aenet_Rc_max, &
+                            nblist=nblist(:,k), nbtype=nbtype(:,k))
+    end do
+    call lcl_final()
+    call system_clock(c1)
+    t_nbl = dble(c1 - c0)/dble(crate)
+    F = 0.0d0
+    Etot = 0.0d0
+    call system_clock(c0)
+    do k = 1, nsub
+       call aenet_atomic_energy_and_forces(cooi(:,k), types(idx(k)), idx(k), nnb(k), &
+            nbcoo(:,1:nnb(k),k), nbtype(1:nnb(k),k), nblist(1:nnb(k),k), natoms, E_i, F, st)
+       if (st /= 0) stat = st
+       Etot = Etot + E_i
+    end do
+    call system_clock(c1)
+    t_eval = dble(c1 - c0)/dble(crate)
+    deallocate(nblist, nbtype, nbcoo, nbdist, cooi, nnb, idx, F)
+
+  end subroutine refdrv_timed
+
 end module refdrv
