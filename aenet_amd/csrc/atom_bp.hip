@@ -1,19 +1,630 @@
-// atom_bp.hip -- Behler-Parrinello atom kernel (not built yet: evaluations
-// of Behler2011 potentials report an unsupported-type error).
+// atom_bp.hip -- per-atom kernel for Behler-Parrinello symmetry functions
+// (G1-G5 of src/ext/symmfunc.f90, the "Behler2011" descriptor).
+//
+// Persistent workgroups; each wavefront evaluates one central atom at a
+// time:
+//   1. neighbors  : compact list of k_search (structures) or the caller's
+//                   list (per-atom entry points); neighbors are bucketed by
+//                   their type local to the central atom's descriptor, so
+//                   that every block of functions -- one per neighbor type
+//                   (radial) and per unordered neighbor-type pair
+//                   (angular), sf_build_index (src/ext/symmfunc.f90:284-305)
+//                   -- is uniform across the wavefront
+//   2. descriptors: sf_fingerprint (src/ext/symmfunc.f90:311-431).  Radial
+//                   functions: lanes = neighbors of the block's type.
+//                   Angular functions: lanes = neighbor pairs (j, k) of the
+//                   block's type pair; pairs that cannot contribute (r_jk
+//                   beyond the block's cutoff, for blocks without G5) are
+//                   dropped by a ballot-compacted queue first, so the
+//                   function loop runs on full wavefronts.  Per-function
+//                   partial sums in registers, reduce-scatter across lanes.
+//                   G4 = 2 [(1+lambda cos)/2]^zeta exp(-eta S) fc fc fc is
+//                   one exp2 per function: 2^(zeta log2(arg) - eta S log2 e)
+//   3. normalize  : stp_normalize (src/sfsetup.f90:985-1035) in FP64
+//   4. network    : shared with the Chebyshev kernel (atom_common.h)
+//   5. forces     : the same traversal again with dE/dG: per pair / triplet
+//                   the function sums collapse to dE/dr_ij, dE/dr_ik,
+//                   dE/dr_jk and dE/dcos, which are turned into per-neighbor
+//                   gradients in LDS, then FP64 atomics into F
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "aenet_amd_internal.h"
+#include "atom_common.h"
 #include "kernels.h"
+#include "kernels_common.h"
 
 namespace aenet_amd {
 
-__global__ void k_bp_unsupported(int *flags) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) atomicOr(flags, AENET_AMD_FLAG_BAD_TYPE);
+namespace {
+
+constexpr float LOG2E = 1.44269504088896341f;
+constexpr int RCH = 16;   // radial functions per register chunk
+constexpr int ACH = 32;   // angular functions per register chunk
+constexpr int QCAP = 128; // triplet queue entries per wavefront
+constexpr int MT = AENET_AMD_MAX_TYPES;
+
+}  // namespace
+
+struct BPWave {
+  float *R;     // [3*cap] R_j - R_i
+  float *r;     // [cap]
+  int *t;       // [cap] local type of the neighbor (0-based; -1: not in the descriptor)
+  int *j;       // [cap] neighbor ids (global atom or environment entry)
+  int *bidx;    // [cap] neighbor slots grouped by local type
+  float *fs;    // [3*cap] per-neighbor gradients dE_i/dR_j
+  int *queue;   // [QCAP] packed (j slot, k slot) of pending triplets
+  int *bstart;  // [MT+1] bucket starts in bidx
+  NetLds net;
+};
+
+struct BPLaunch {
+  int natoms;    // central atoms (structures) or environments
+  int cap;       // neighbor capacity of the LDS region
+  int nsf;       // max features
+  int nl;        // max layers
+  int wsize;     // total weights
+  int w_in_lds;  // weights staged in LDS
+  int waves;     // wavefronts per workgroup
+};
+
+__host__ __device__ inline int bp_wave_words(const BPLaunch &P) {
+  return 10 * P.cap + QCAP + 16 + net_lds_words(P.nsf, P.nl);
 }
 
-void launch_atom_bp(int, bool, const AtomSizes &, const ModelDev &, const BatchDev &b, const NbDev &,
-                    const OutDev &, const EnvDev &, hipStream_t stream) {
-  hipLaunchKernelGGL(k_bp_unsupported, dim3(1), dim3(64), 0, stream, b.flags);
+__device__ inline BPWave bp_carve(float *base, const BPLaunch &P) {
+  const int cap = P.cap;
+  BPWave w;
+  w.R = base;
+  w.r = w.R + 3 * cap;
+  w.t = reinterpret_cast<int *>(w.r + cap);
+  w.j = w.t + cap;
+  w.bidx = w.j + cap;
+  w.fs = reinterpret_cast<float *>(w.bidx + cap);
+  w.queue = reinterpret_cast<int *>(w.fs + 3 * cap);
+  w.bstart = w.queue + QCAP;
+  w.net = carve_net(reinterpret_cast<float *>(w.bstart + 16), P.nsf, P.nl);
+  return w;
+}
+
+// value and radial derivative of one G1/G2/G3 function at distance r
+// (sf_G1_ij, sf_G2_ij, sf_G3_ij, src/ext/symmfunc.f90:464-580)
+__device__ __forceinline__ void radial_fn(const BPFunc &f, float r, float fc, float dfc, float &v, float &dv) {
+  if (f.kind == 2) {
+    const float a = r - f.p1;
+    const float e = exp2f(-f.p2 * LOG2E * a * a);
+    v = e * fc;
+    dv = e * (dfc - 2.0f * f.p2 * a * fc);
+  } else if (f.kind == 3) {
+    float s, c;
+    sincosf(f.p1 * r, &s, &c);
+    v = c * fc;
+    dv = c * dfc - f.p1 * s * fc;
+  } else {
+    v = fc;
+    dv = dfc;
+  }
+}
+
+// geometry of one triplet (i; j, k): unit vectors, distances, cos(theta)
+// clamped to [-1, 1] (src/ext/symmfunc.f90:409-411)
+struct Trip {
+  float uj[3], uk[3], d[3];  // d = R_k - R_j
+  float rj, rk, rjk, cosv;
+};
+
+__device__ __forceinline__ Trip make_trip(const float *R, const float *r, int ja, int kb) {
+  Trip T;
+  T.rj = r[ja];
+  T.rk = r[kb];
+  const float ij = 1.0f / T.rj, ik = 1.0f / T.rk;
+  float dot = 0.0f, d2 = 0.0f;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    const float a = R[3 * ja + x], b = R[3 * kb + x];
+    T.uj[x] = a * ij;
+    T.uk[x] = b * ik;
+    T.d[x] = b - a;
+    d2 += T.d[x] * T.d[x];
+  }
+  dot = T.uj[0] * T.uk[0] + T.uj[1] * T.uk[1] + T.uj[2] * T.uk[2];
+  T.cosv = fminf(1.0f, fmaxf(-1.0f, dot));
+  T.rjk = sqrtf(d2);
+  return T;
+}
+
+// Angular traversal of one block: every unordered neighbor pair (j, k) of
+// local types (t2, t3) that can contribute goes through `op` with full
+// wavefronts.  Candidates are decoded from the bucket pair, filtered by
+// r_ij, r_ik < rc and (unless keep_all: blocks with G5) r_jk < rc, where
+// rc is the block's largest cutoff, compacted into the LDS queue, and
+// handed out 64 at a time.
+template <class Op>
+__device__ __forceinline__ void for_triplets(const BPWave &S, int b2, int m2, int b3, int m3, bool same, bool keep_all,
+                                             float rc2, Op &op) {
+  const int lane = lane_id();
+  const int ncand = same ? m2 * (m2 - 1) / 2 : m2 * m3;
+  const float inv_m3 = m3 > 0 ? 1.0f / (float)m3 : 0.0f;
+  int qn = 0;
+  for (int base = 0; base < ncand; base += WAVE) {
+    const int p = base + lane;
+    bool ok = false;
+    int packed = 0;
+    if (p < ncand) {
+      int a, c;
+      if (same) {
+        tri_decode(p, m2, a, c);
+      } else {
+        a = (int)(((float)p + 0.5f) * inv_m3);
+        c = p - a * m3;
+        if (c < 0) {
+          --a;
+          c += m3;
+        } else if (c >= m3) {
+          ++a;
+          c -= m3;
+        }
+      }
+      const int ja = S.bidx[b2 + a], kb = S.bidx[b3 + c];
+      const float rj = S.r[ja], rk = S.r[kb];
+      if (!(rj * rj < rc2 && rk * rk < rc2)) {
+        ok = false;  // fc(r_ij) fc(r_ik) = 0 for every function of the block
+      } else if (keep_all) {
+        ok = true;
+      } else {
+        const float dx = S.R[3 * kb] - S.R[3 * ja], dy = S.R[3 * kb + 1] - S.R[3 * ja + 1],
+                    dz = S.R[3 * kb + 2] - S.R[3 * ja + 2];
+        ok = dx * dx + dy * dy + dz * dz < rc2;
+      }
+      packed = ja | (kb << 16);
+    }
+    const unsigned long long m = wave_ballot(ok);
+    if (ok) S.queue[qn + mask_prefix(m)] = packed;
+    qn += popc64(m);
+    wave_sync();
+    if (qn >= WAVE) {
+      const int e = S.queue[lane];
+      op(e & 0xffff, e >> 16, true);
+      wave_sync();
+      if (lane < qn - WAVE) S.queue[lane] = S.queue[WAVE + lane];
+      qn -= WAVE;
+      wave_sync();
+    }
+  }
+  if (qn > 0) {
+    const int e = lane < qn ? S.queue[lane] : 0;
+    op(e & 0xffff, e >> 16, lane < qn);
+    wave_sync();
+  }
+}
+
+// cutoffs of the three legs of a triplet for one cutoff radius
+struct TripCut {
+  float rc;
+  float cj, dcj, ck, dck, cjk, dcjk;
+};
+
+__device__ __forceinline__ void trip_cut(const Trip &T, float rc, TripCut &c) {
+  const float inv = 1.0f / rc;
+  c.rc = rc;
+  cutoff(T.rj, rc, inv, c.cj, c.dcj);
+  cutoff(T.rk, rc, inv, c.ck, c.dck);
+  cutoff(T.rjk, rc, inv, c.cjk, c.dcjk);
+}
+
+// energy pass of an angular chunk: G_f += 2 arg^zeta exp(-eta S) C
+struct AngValueOp {
+  const float *R, *r;
+  const BPFunc *fn;
+  int nf;  // functions in this chunk
+  double acc[ACH];
+  __device__ __forceinline__ void operator()(int ja, int kb, bool valid) {
+    if (!valid) return;
+    const Trip T = make_trip(R, r, ja, kb);
+    const float argp = 0.5f * (1.0f + T.cosv), argm = 0.5f * (1.0f - T.cosv);
+    const float Lp = log2f(fmaxf(argp, 1.0e-30f)), Lm = log2f(fmaxf(argm, 1.0e-30f));
+    const float S2 = T.rj * T.rj + T.rk * T.rk, S3 = S2 + T.rjk * T.rjk;
+    TripCut c;
+    c.rc = -1.0f;
+#pragma unroll
+    for (int k = 0; k < ACH; ++k) {
+      if (k < nf) {
+        const BPFunc f = fn[k];
+        if (f.Rc != c.rc) trip_cut(T, f.Rc, c);
+        const float lam = f.p1, zeta = f.p2, eta = f.p3;
+        float arg, L;
+        if (lam == 1.0f) {
+          arg = argp;
+          L = Lp;
+        } else if (lam == -1.0f) {
+          arg = argm;
+          L = Lm;
+        } else {
+          arg = 0.5f * (1.0f + lam * T.cosv);
+          L = log2f(fmaxf(arg, 1.0e-30f));
+        }
+        const bool g4 = f.kind == 4;
+        const float P = 2.0f * exp2f((zeta - 1.0f) * L - eta * LOG2E * (g4 ? S3 : S2));
+        const float C = g4 ? c.cj * c.ck * c.cjk : c.cj * c.ck;
+        acc[k] += (double)(P * arg * C);
+      }
+    }
+  }
+};
+
+// force pass of an angular block: dE/d(r_ij, r_ik, r_jk, cos) from every
+// function of the block, then the per-neighbor gradients
+struct AngForceOp {
+  const float *R, *r;
+  float *fs;
+  const BPFunc *fn;
+  const float *coef;  // dE/dG of the block's functions
+  int nf;
+  __device__ __forceinline__ void operator()(int ja, int kb, bool valid) {
+    if (!valid) return;
+    const Trip T = make_trip(R, r, ja, kb);
+    const float argp = 0.5f * (1.0f + T.cosv), argm = 0.5f * (1.0f - T.cosv);
+    const float Lp = log2f(fmaxf(argp, 1.0e-30f)), Lm = log2f(fmaxf(argm, 1.0e-30f));
+    const float S2 = T.rj * T.rj + T.rk * T.rk, S3 = S2 + T.rjk * T.rjk;
+    float dRj = 0.0f, dRk = 0.0f, dRjk = 0.0f, dcos = 0.0f;
+    // sums over a segment of functions with the same kind and cutoff
+    float s_cos = 0.0f, s_E = 0.0f, s_eta = 0.0f;
+    TripCut c;
+    c.rc = -1.0f;
+    bool seg_g4 = true;
+    auto flush = [&]() {
+      const float cjk = seg_g4 ? c.cjk : 1.0f;
+      const float C = c.cj * c.ck * cjk;
+      dcos += s_cos * C;
+      dRj += s_E * (c.dcj * c.ck * cjk) + s_eta * T.rj * C;
+      dRk += s_E * (c.cj * c.dck * cjk) + s_eta * T.rk * C;
+      if (seg_g4) dRjk += s_E * (c.cj * c.ck * c.dcjk) + s_eta * T.rjk * C;
+      s_cos = s_E = s_eta = 0.0f;
+    };
+    for (int k = 0; k < nf; ++k) {
+      const BPFunc f = fn[k];
+      const bool g4 = f.kind == 4;
+      if (f.Rc != c.rc || g4 != seg_g4) {
+        if (c.rc > 0.0f) flush();
+        if (f.Rc != c.rc) trip_cut(T, f.Rc, c);
+        seg_g4 = g4;
+      }
+      const float lam = f.p1, zeta = f.p2, eta = f.p3;
+      float arg, L;
+      if (lam == 1.0f) {
+        arg = argp;
+        L = Lp;
+      } else if (lam == -1.0f) {
+        arg = argm;
+        L = Lm;
+      } else {
+        arg = 0.5f * (1.0f + lam * T.cosv);
+        L = log2f(fmaxf(arg, 1.0e-30f));
+      }
+      const float P = 2.0f * exp2f((zeta - 1.0f) * L - eta * LOG2E * (g4 ? S3 : S2));
+      const float w = coef[k];
+      const float wE = w * P * arg;
+      s_cos += w * (0.5f * zeta * lam) * P;
+      s_E += wE;
+      s_eta += wE * (-2.0f * eta);
+    }
+    if (c.rc > 0.0f) flush();
+    const float inv_rjk = T.rjk > 1.0e-8f ? 1.0f / T.rjk : 1.0f;
+    const float cj = dcos / T.rj, ck = dcos / T.rk;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      const float e = dRjk * T.d[x] * inv_rjk;
+      const float gj = dRj * T.uj[x] + cj * (T.uk[x] - T.cosv * T.uj[x]) - e;
+      const float gk = dRk * T.uk[x] + ck * (T.uj[x] - T.cosv * T.uk[x]) + e;
+      atomicAdd(&fs[3 * ja + x], gj);
+      atomicAdd(&fs[3 * kb + x], gk);
+    }
+  }
+};
+
+// MODE 0: structures (compact lists, FP64 atomic force scatter)
+// MODE 1: explicit environments (per-neighbor forces out)
+template <int MODE, bool FORCES>
+__global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, BatchDev b, NbDev nbl, OutDev o,
+                                                                EnvDev env, BPLaunch P) {
+  extern __shared__ float lds[];
+  const int lane = lane_id();
+  const int wave = threadIdx.x / WAVE;
+  const int nw = P.waves;
+  const float *Wsrc = md.W;
+  if (P.w_in_lds) {
+    for (int k = threadIdx.x; k < P.wsize; k += 64 * nw) lds[k] = md.W[k];
+    __syncthreads();
+    Wsrc = lds;
+  }
+  const int wsz = P.w_in_lds ? (P.wsize + 3) / 4 * 4 : 0;
+  BPWave S = bp_carve(lds + wsz + wave * bp_wave_words(P), P);
+  const BPDev &bp = *md.bp;
+  const int nt = bp.ntypes;
+
+  for (int g = blockIdx.x * nw + wave; g < P.natoms; g += gridDim.x * nw) {
+    wave_sync();
+    // ------------------------------------------------------------ 1
+    int ti, n;
+    if (MODE == 0) {
+      ti = b.types[g];
+      n = min(nbl.count[g], P.cap);
+    } else {
+      ti = env.env_type[g];
+      n = env.env_start[g + 1] - env.env_start[g];
+      if (n > P.cap) {
+        if (lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_NB_OVERFLOW);
+        n = P.cap;
+      }
+    }
+    ti = uniform(ti);
+    n = uniform(n);
+    if (ti < 1 || ti > md.ntypes) {
+      if (lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_BAD_TYPE);
+      continue;
+    }
+    const SpeciesDev &sp = md.species[ti - 1];
+    bool bad_nb = false;
+    if (MODE == 0) {
+      const float4 *R4 = reinterpret_cast<const float4 *>(nbl.R) + (size_t)g * nbl.cap;
+      const int *J = nbl.j + (size_t)g * nbl.cap;
+      for (int q = lane; q < n; q += WAVE) {
+        float4 v = R4[q];
+        int j = J[q];
+        int tj = b.types[j];
+        int lt = (tj >= 1 && tj <= md.ntypes) ? sp.bp_ltype[tj - 1] - 1 : -1;
+        bad_nb |= lt < 0;
+        S.R[3 * q] = v.x;
+        S.R[3 * q + 1] = v.y;
+        S.R[3 * q + 2] = v.z;
+        S.r[q] = v.w;
+        S.j[q] = j;
+        S.t[q] = lt;
+        S.fs[3 * q] = 0.0f;
+        S.fs[3 * q + 1] = 0.0f;
+        S.fs[3 * q + 2] = 0.0f;
+      }
+    } else {
+      const int e0 = env.env_start[g];
+      const double ci[3] = {env.coo_i[3 * g], env.coo_i[3 * g + 1], env.coo_i[3 * g + 2]};
+      for (int q = lane; q < n; q += WAVE) {
+        double c[3];
+        for (int k = 0; k < 3; ++k) c[k] = env.coo_j[3 * (size_t)(e0 + q) + k] - ci[k];
+        int tj = env.type_j[e0 + q];
+        int lt = (tj >= 1 && tj <= md.ntypes) ? sp.bp_ltype[tj - 1] - 1 : -1;
+        bad_nb |= lt < 0;
+        S.R[3 * q] = (float)c[0];
+        S.R[3 * q + 1] = (float)c[1];
+        S.R[3 * q + 2] = (float)c[2];
+        S.r[q] = (float)sqrt(dist2_exact(c));
+        S.j[q] = e0 + q;
+        S.t[q] = lt;
+        S.fs[3 * q] = 0.0f;
+        S.fs[3 * q + 1] = 0.0f;
+        S.fs[3 * q + 2] = 0.0f;
+      }
+    }
+    if (wave_ballot(bad_nb) != 0ull && lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_BAD_TYPE);
+    wave_sync();
+    // buckets by local type
+    {
+      int cnt = 0;
+      for (int t = 0; t < nt; ++t) {
+        if (lane == 0) S.bstart[t] = cnt;
+        for (int base = 0; base < n; base += WAVE) {
+          const int q = base + lane;
+          const bool in = q < n && S.t[q] == t;
+          const unsigned long long m = wave_ballot(in);
+          if (in) S.bidx[cnt + mask_prefix(m)] = q;
+          cnt += popc64(m);
+        }
+      }
+      if (lane == 0) S.bstart[nt] = cnt;
+    }
+    wave_sync();
+    const int row = sp.bp_row;
+
+    // ------------------------------------------------------------ 2 + 3
+    // radial blocks
+    for (int t2 = 0; t2 < nt; ++t2) {
+      const int key2 = row * MT + t2;
+      const int nf = bp.rad_n[key2], off = bp.rad_off[key2];
+      const BPFunc *fn = md.bpf + bp.rad_p0[key2];
+      const int bs = S.bstart[t2], m = S.bstart[t2 + 1] - bs;
+      for (int c0 = 0; c0 < nf; c0 += RCH) {
+        float acc[RCH];
+#pragma unroll
+        for (int k = 0; k < RCH; ++k) acc[k] = 0.0f;
+        for (int q = lane; q < m; q += WAVE) {
+          const float r = S.r[S.bidx[bs + q]];
+          float rc = -1.0f, fc = 0.0f, dfc = 0.0f;
+#pragma unroll
+          for (int k = 0; k < RCH; ++k) {
+            if (c0 + k < nf) {
+              const BPFunc f = fn[c0 + k];
+              if (f.Rc != rc) {
+                rc = f.Rc;
+                cutoff(r, rc, 1.0f / rc, fc, dfc);
+              }
+              float v, dv;
+              radial_fn(f, r, fc, dfc, v, dv);
+              acc[k] += v;
+            }
+          }
+        }
+        const double G = reduce_scatter<RCH>(acc);
+        const int idx = lane >> 2;
+        if ((lane & 3) == 0 && c0 + idx < nf) {
+          const int ft = off + c0 + idx;
+          const double Gs = md.norm_scale[sp.norm_off + ft] * (G - md.norm_avg[sp.norm_off + ft]);
+          S.net.gp[ft] = (float)Gs;
+          if (o.G) o.G[(size_t)g * o.G_stride + ft] = o.G_scaled ? Gs : G;
+        }
+      }
+    }
+    // angular blocks, unordered type pairs t2 <= t3
+    for (int t2 = 0; t2 < nt; ++t2) {
+      for (int t3 = t2; t3 < nt; ++t3) {
+        const int key3 = (row * MT + t2) * MT + t3;
+        const int nf = bp.ang_n[key3], off = bp.ang_off[key3];
+        if (nf == 0) continue;
+        const BPFunc *fn = md.bpf + bp.ang_p0[key3];
+        const int b2 = S.bstart[t2], m2 = S.bstart[t2 + 1] - b2;
+        const int b3 = S.bstart[t3], m3 = S.bstart[t3 + 1] - b3;
+        const float rc = bp.ang_rcmax[key3];
+        for (int c0 = 0; c0 < nf; c0 += ACH) {
+          AngValueOp op;
+          op.R = S.R;
+          op.r = S.r;
+          op.fn = fn + c0;
+          op.nf = min(ACH, nf - c0);
+#pragma unroll
+          for (int k = 0; k < ACH; ++k) op.acc[k] = 0.0;
+          for_triplets(S, b2, m2, b3, m3, t2 == t3, bp.ang_g5[key3] != 0, rc * rc, op);
+          const double G = reduce_scatter<ACH>(op.acc);
+          const int idx = lane >> 1;
+          if ((lane & 1) == 0 && c0 + idx < nf) {
+            const int ft = off + c0 + idx;
+            const double Gs = md.norm_scale[sp.norm_off + ft] * (G - md.norm_avg[sp.norm_off + ft]);
+            S.net.gp[ft] = (float)Gs;
+            if (o.G) o.G[(size_t)g * o.G_stride + ft] = o.G_scaled ? Gs : G;
+          }
+        }
+      }
+    }
+    wave_sync();
+    if (o.desc_only) continue;
+
+    // ------------------------------------------------------------ 4
+    const float yout = network(md, sp, Wsrc, S.net, FORCES);
+    if (lane == 0) {
+      if (MODE == 0)
+        o.E_atom[g] = atom_energy(sp, yout);
+      else
+        env.E[g] = atom_energy(sp, yout);
+    }
+    if (!FORCES) continue;
+
+    // ------------------------------------------------------------ 5
+    const float *coef = S.net.coef;
+    for (int t2 = 0; t2 < nt; ++t2) {
+      const int key2 = row * MT + t2;
+      const int nf = bp.rad_n[key2], off = bp.rad_off[key2];
+      if (nf == 0) continue;
+      const BPFunc *fn = md.bpf + bp.rad_p0[key2];
+      const int bs = S.bstart[t2], m = S.bstart[t2 + 1] - bs;
+      for (int q = lane; q < m; q += WAVE) {
+        const int nq = S.bidx[bs + q];
+        const float r = S.r[nq];
+        float rc = -1.0f, fc = 0.0f, dfc = 0.0f, s = 0.0f;
+        for (int k = 0; k < nf; ++k) {
+          const BPFunc f = fn[k];
+          if (f.Rc != rc) {
+            rc = f.Rc;
+            cutoff(r, rc, 1.0f / rc, fc, dfc);
+          }
+          float v, dv;
+          radial_fn(f, r, fc, dfc, v, dv);
+          s += coef[off + k] * dv;
+        }
+        // each neighbor sits in exactly one bucket: one writer per slot
+        const float sr = s / r;
+        S.fs[3 * nq] += sr * S.R[3 * nq];
+        S.fs[3 * nq + 1] += sr * S.R[3 * nq + 1];
+        S.fs[3 * nq + 2] += sr * S.R[3 * nq + 2];
+      }
+    }
+    wave_sync();
+    for (int t2 = 0; t2 < nt; ++t2) {
+      for (int t3 = t2; t3 < nt; ++t3) {
+        const int key3 = (row * MT + t2) * MT + t3;
+        const int nf = bp.ang_n[key3], off = bp.ang_off[key3];
+        if (nf == 0) continue;
+        const int b2 = S.bstart[t2], m2 = S.bstart[t2 + 1] - b2;
+        const int b3 = S.bstart[t3], m3 = S.bstart[t3 + 1] - b3;
+        const float rc = bp.ang_rcmax[key3];
+        AngForceOp op;
+        op.R = S.R;
+        op.r = S.r;
+        op.fs = S.fs;
+        op.fn = md.bpf + bp.ang_p0[key3];
+        op.coef = coef + off;
+        op.nf = nf;
+        for_triplets(S, b2, m2, b3, m3, t2 == t3, bp.ang_g5[key3] != 0, rc * rc, op);
+      }
+    }
+    wave_sync();
+    // F_j -= dE_i/dR_j ; F_i += sum_j dE_i/dR_j  (src/aenet.f90:471-477)
+    double own[3] = {0.0, 0.0, 0.0};
+    for (int q = lane; q < n; q += WAVE) {
+      double gx = S.fs[3 * q], gy = S.fs[3 * q + 1], gz = S.fs[3 * q + 2];
+      own[0] += gx;
+      own[1] += gy;
+      own[2] += gz;
+      if (MODE == 0) {
+        int j = S.j[q];
+        unsafeAtomicAdd(&o.F[3 * (size_t)j], -gx);
+        unsafeAtomicAdd(&o.F[3 * (size_t)j + 1], -gy);
+        unsafeAtomicAdd(&o.F[3 * (size_t)j + 2], -gz);
+      } else {
+        size_t e = (size_t)S.j[q];
+        env.dF_j[3 * e] = -gx;
+        env.dF_j[3 * e + 1] = -gy;
+        env.dF_j[3 * e + 2] = -gz;
+      }
+    }
+    for (int x = 0; x < 3; ++x) own[x] = wave_sum(own[x]);
+    if (lane == 0) {
+      if (MODE == 0) {
+        unsafeAtomicAdd(&o.F[3 * (size_t)g], own[0]);
+        unsafeAtomicAdd(&o.F[3 * (size_t)g + 1], own[1]);
+        unsafeAtomicAdd(&o.F[3 * (size_t)g + 2], own[2]);
+      } else {
+        env.dF_i[3 * g] = own[0];
+        env.dF_i[3 * g + 1] = own[1];
+        env.dF_i[3 * g + 2] = own[2];
+      }
+    }
+  }
+}
+
+template <int MODE, bool F>
+static void launch_bp(const ModelDev &md, const BatchDev &b, const NbDev &nb, const OutDev &o, const EnvDev &e,
+                      BPLaunch P, hipStream_t stream) {
+  PersistentShape ps;
+  const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
+  if (!persistent_shape(wbytes, 4 * (size_t)bp_wave_words(P), P.natoms, ps)) {
+    raise_flag(b.flags, AENET_AMD_FLAG_NB_OVERFLOW, stream);
+    return;
+  }
+  P.w_in_lds = ps.w_in_lds;
+  P.waves = ps.waves;
+  hipLaunchKernelGGL((k_atom_bp<MODE, F>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream, md, b, nb, o, e, P);
+}
+
+void launch_atom_bp(int mode, bool forces, const AtomSizes &z, const ModelDev &md, const BatchDev &b,
+                    const NbDev &nb, const OutDev &o, const EnvDev &e, hipStream_t stream) {
+  if (z.natoms <= 0) return;
+  BPLaunch P;
+  P.natoms = z.natoms;
+  P.cap = z.cap;
+  P.nsf = z.nsf_max;
+  P.nl = std::max(z.nl_max, 2);
+  P.wsize = z.wsize;
+  P.w_in_lds = 0;
+  P.waves = 1;
+  if (mode == 0) {
+    if (forces)
+      launch_bp<0, true>(md, b, nb, o, e, P, stream);
+    else
+      launch_bp<0, false>(md, b, nb, o, e, P, stream);
+  } else {
+    if (forces)
+      launch_bp<1, true>(md, b, nb, o, e, P, stream);
+    else
+      launch_bp<1, false>(md, b, nb, o, e, P, stream);
+  }
 }
 
 }  // namespace aenet_amd

@@ -24,126 +24,24 @@
 #include "aenet_amd_internal.h"
 #include "kernels.h"
 #include "kernels_common.h"
+#include "atom_common.h"
 
 namespace aenet_amd {
 
 namespace {
-
 constexpr float EPS_R = 1.0e-12f;
-constexpr int MAX_WAVES = 4;  // wavefronts per workgroup (fewer when LDS is short)
-
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ void activate(int f, float x, float &y, float &dy) {
-  // ff_activate (src/ext/feedforward.f90:932-969)
-  switch (f) {
-    case 0: y = x; dy = 1.0f; break;
-    case 1: y = tanhf(x); dy = 1.0f - y * y; break;
-    case 2: y = 1.0f / (1.0f + expf(-x)); dy = y * (1.0f - y); break;
-    case 3: { float t = tanhf(0.666666666666667f * x); y = 1.7159f * t; dy = 1.7159f * (1.0f - t * t) * 0.666666666666667f; } break;
-    case 4: { float t = tanhf(0.666666666666667f * x); y = 1.7159f * t + 0.1f * x; dy = 1.7159f * (1.0f - t * t) * 0.666666666666667f + 0.1f; } break;
-    default: y = 0.0f; dy = 0.0f;
-  }
-}
-
-// cutoff function and its derivative (sfb_fc, sfb_fc_d1,
-// src/ext/sfbasis.f90:719-752)
-__device__ __forceinline__ void cutoff(float r, float Rc, float inv_Rc, float &fc, float &dfc) {
-  if (r >= Rc) {
-    fc = 0.0f;
-    dfc = 0.0f;
-  } else {
-    float sp, cp;
-    sincospif(r * inv_Rc, &sp, &cp);
-    fc = 0.5f * (cp + 1.0f);
-    dfc = -0.5f * 3.14159265358979f * inv_Rc * sp;
-  }
-}
-
-// triangular index p -> (a, b), 0 <= a < b < n, row-major
-__device__ __forceinline__ void tri_decode(int p, int n, int &a, int &b) {
-  float nn = (float)(2 * n - 1);
-  int r = (int)((nn - sqrtf(nn * nn - 8.0f * (float)p)) * 0.5f);
-  if (r < 0) r = 0;
-  if (r > n - 2) r = n - 2;
-  while (r > 0 && r * n - r * (r + 1) / 2 > p) --r;
-  while ((r + 1) * n - (r + 1) * (r + 2) / 2 <= p) ++r;
-  a = r;
-  b = p - (r * n - r * (r + 1) / 2) + r + 1;
-}
-
-// Reduce-scatter of 64 per-lane values over the wavefront: on return lane
-// f holds the sum over all lanes of v[f].  First level in FP32 (pairs of
-// partial sums), the remaining five levels in FP64.
-__device__ __forceinline__ double reduce_scatter64(float (&v)[64]) {
-  const int lane = lane_id();
-  float h[32];
-#pragma unroll
-  for (int k = 0; k < 32; ++k) {
-    bool up = lane & 32;
-    float send = up ? v[k] : v[k + 32];
-    float keep = up ? v[k + 32] : v[k];
-    h[k] = keep + __shfl_xor(send, 32, WAVE);
-  }
-  double d[16];
-#pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    bool up = lane & 16;
-    double send = up ? (double)h[k] : (double)h[k + 16];
-    double keep = up ? (double)h[k + 16] : (double)h[k];
-    d[k] = keep + __shfl_xor(send, 16, WAVE);
-  }
-#pragma unroll
-  for (int k = 0; k < 8; ++k) {
-    bool up = lane & 8;
-    double send = up ? d[k] : d[k + 8];
-    double keep = up ? d[k + 8] : d[k];
-    d[k] = keep + __shfl_xor(send, 8, WAVE);
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    bool up = lane & 4;
-    double send = up ? d[k] : d[k + 4];
-    double keep = up ? d[k + 4] : d[k];
-    d[k] = keep + __shfl_xor(send, 4, WAVE);
-  }
-#pragma unroll
-  for (int k = 0; k < 2; ++k) {
-    bool up = lane & 2;
-    double send = up ? d[k] : d[k + 2];
-    double keep = up ? d[k + 2] : d[k];
-    d[k] = keep + __shfl_xor(send, 2, WAVE);
-  }
-  {
-    bool up = lane & 1;
-    double send = up ? d[0] : d[1];
-    double keep = up ? d[1] : d[0];
-    d[0] = keep + __shfl_xor(send, 1, WAVE);
-  }
-  return d[0];
-}
-
 }  // namespace
 
 // Per-wavefront LDS region, laid out after the weights in dynamic shared
-// memory; cap = neighbor capacity, nsf = max features.
+// memory; cap = neighbor capacity.
 struct WaveLds {
-  float *R;      // [3*cap]
-  float *r;      // [cap]
-  int *t;        // [cap] neighbor types
-  int *j;        // [cap] neighbor ids (global atom or environment entry)
-  int *ang;      // [cap] angular neighbor list
-  float *fs;     // [3*cap] per-neighbor forces
-  float *gp;     // [nsf] normalized descriptors
-  float *coef;   // [nsf] dE/dG
-  float *d0;     // [dmax] back-propagation buffers, dmax = max(nsf, 64)
-  float *d1;     // [dmax]
-  float *act;    // [nl*64] activations of layer l at act[64*l]
-  float *dact;   // [nl*64]
+  float *R;    // [3*cap]
+  float *r;    // [cap]
+  int *t;      // [cap] neighbor types
+  int *j;      // [cap] neighbor ids (global atom or environment entry)
+  int *ang;    // [cap] angular neighbor list
+  float *fs;   // [3*cap] per-neighbor forces
+  NetLds net;  // network stage
 };
 
 struct ChebLaunch {
@@ -156,16 +54,10 @@ struct ChebLaunch {
   int waves;     // wavefronts per workgroup
 };
 
-__host__ __device__ inline int dmax_words(int nsf) {
-  return nsf > AENET_AMD_MAX_HIDDEN ? nsf : AENET_AMD_MAX_HIDDEN;
-}
-
-__host__ __device__ inline int wave_lds_words(const ChebLaunch &P) {
-  return 10 * P.cap + 2 * P.nsf + 2 * dmax_words(P.nsf) + 2 * P.nl * AENET_AMD_MAX_HIDDEN;
-}
+__host__ __device__ inline int wave_lds_words(const ChebLaunch &P) { return 10 * P.cap + net_lds_words(P.nsf, P.nl); }
 
 __device__ inline WaveLds carve(float *base, const ChebLaunch &P) {
-  const int cap = P.cap, nsf = P.nsf, dm = dmax_words(P.nsf);
+  const int cap = P.cap;
   WaveLds w;
   w.R = base;
   w.r = w.R + 3 * cap;
@@ -173,19 +65,14 @@ __device__ inline WaveLds carve(float *base, const ChebLaunch &P) {
   w.j = w.t + cap;
   w.ang = w.j + cap;
   w.fs = reinterpret_cast<float *>(w.ang + cap);
-  w.gp = w.fs + 3 * cap;
-  w.coef = w.gp + nsf;
-  w.d0 = w.coef + nsf;
-  w.d1 = w.d0 + dm;
-  w.act = w.d1 + dm;
-  w.dact = w.act + P.nl * AENET_AMD_MAX_HIDDEN;
+  w.net = carve_net(w.fs + 3 * cap, P.nsf, P.nl);
   return w;
 }
 
 // MODE 0: structures (compact lists, FP64 atomic force scatter)
 // MODE 1: explicit environments (per-neighbor forces out)
 template <int RN, int AN, int MODE, bool FORCES>
-__global__ __launch_bounds__(64 * MAX_WAVES) void k_atom_cheb(ModelDev md, BatchDev b, NbDev nbl, OutDev o, EnvDev env,
+__global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_cheb(ModelDev md, BatchDev b, NbDev nbl, OutDev o, EnvDev env,
                                                           ChebLaunch P) {
   static_assert(2 * (RN + AN) <= 64, "features of one chunk");
   extern __shared__ float lds[];
@@ -320,7 +207,7 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_atom_cheb(ModelDev md, Batch
         T = Tn;
       }
     }
-    double G = reduce_scatter64(acc);
+    double G = reduce_scatter<64>(acc);
     // lane f holds template feature f; map it to the species' layout
     {
       int f = -1;
@@ -336,7 +223,7 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_atom_cheb(ModelDev md, Batch
       if (f >= 0) {
         // ------------------------------------------------------ 3
         double Gs = md.norm_scale[sp.norm_off + f] * (G - md.norm_avg[sp.norm_off + f]);
-        S.gp[f] = (float)Gs;
+        S.net.gp[f] = (float)Gs;
         if (o.G) o.G[(size_t)g * o.G_stride + f] = o.G_scaled ? Gs : G;
       }
     }
@@ -344,71 +231,19 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_atom_cheb(ModelDev md, Batch
     if (o.desc_only) continue;
 
     // ------------------------------------------------------------ 4
-    const int nl = sp.nlayers;
-    const float *in = S.gp;
-    for (int l = 0; l + 1 < nl; ++l) {
-      const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
-      const float *Wl = Wsrc + sp.w_off[l];
-      if (lane < n2) {
-        float z0 = Wl[n1 * n2 + lane], z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;  // bias column
-        int c = 0;
-        for (; c + 3 < n1; c += 4) {
-          z0 += Wl[c * n2 + lane] * in[c];
-          z1 += Wl[(c + 1) * n2 + lane] * in[c + 1];
-          z2 += Wl[(c + 2) * n2 + lane] * in[c + 2];
-          z3 += Wl[(c + 3) * n2 + lane] * in[c + 3];
-        }
-        for (; c < n1; ++c) z0 += Wl[c * n2 + lane] * in[c];
-        float y, dy;
-        activate(sp.fa[l + 1], (z0 + z1) + (z2 + z3), y, dy);
-        S.act[(l + 1) * AENET_AMD_MAX_HIDDEN + lane] = y;
-        S.dact[(l + 1) * AENET_AMD_MAX_HIDDEN + lane] = dy;
-      }
-      wave_sync();
-      in = S.act + (l + 1) * AENET_AMD_MAX_HIDDEN;
-    }
-    const float yout = S.act[(nl - 1) * AENET_AMD_MAX_HIDDEN];
-    float *dcur = S.d0;
-    float *dnext = S.d1;
-    if (lane == 0) dcur[0] = S.dact[(nl - 1) * AENET_AMD_MAX_HIDDEN];
-    wave_sync();
-    for (int l = nl - 2; l >= 0; --l) {
-      const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
-      const float *Wl = Wsrc + sp.w_off[l];
-      for (int c = lane; c < n1; c += WAVE) {
-        const float *row = Wl + c * n2;
-        float s0 = 0.0f, s1 = 0.0f;
-        int hh = 0;
-        for (; hh + 1 < n2; hh += 2) {
-          s0 += row[hh] * dcur[hh];
-          s1 += row[hh + 1] * dcur[hh + 1];
-        }
-        if (hh < n2) s0 += row[hh] * dcur[hh];
-        float s = s0 + s1;
-        dnext[c] = l > 0 ? s * S.dact[l * AENET_AMD_MAX_HIDDEN + c] : s;
-      }
-      wave_sync();
-      float *t = dcur;
-      dcur = dnext;
-      dnext = t;
-    }
-    const float Es = (float)sp.E_scale;
-    for (int f = lane; f < nsf; f += WAVE) S.coef[f] = Es * dcur[f] * (float)md.norm_scale[sp.norm_off + f];
+    const float yout = network(md, sp, Wsrc, S.net, FORCES);
     if (lane == 0) {
-      double E = sp.E_scale * (double)yout + sp.E_shift;
-      E = E + sp.E_atom;
       if (MODE == 0)
-        o.E_atom[g] = E;
+        o.E_atom[g] = atom_energy(sp, yout);
       else
-        env.E[g] = E;
+        env.E[g] = atom_energy(sp, yout);
     }
     if (!FORCES) continue;
-    wave_sync();
 
     // ------------------------------------------------------------ 5
     float *fs = S.fs;
-    const float *cr1 = S.coef + sp.r_i1, *cr2 = S.coef + sp.r_i2;
-    const float *ca1 = S.coef + sp.a_i1, *ca2 = S.coef + sp.a_i2;
+    const float *cr1 = S.net.coef + sp.r_i1, *cr2 = S.net.coef + sp.r_i2;
+    const float *ca1 = S.net.coef + sp.a_i1, *ca2 = S.net.coef + sp.a_i2;
     for (int q = lane; q < n; q += WAVE) {
       float gx = 0.0f, gy = 0.0f, gz = 0.0f;
       float r = S.r[q];
@@ -530,27 +365,17 @@ __global__ __launch_bounds__(64 * MAX_WAVES) void k_atom_cheb(ModelDev md, Batch
 template <int RN, int AN, int MODE, bool F>
 static void launch_one(const ModelDev &md, const BatchDev &b, const NbDev &nb, const OutDev &o, const EnvDev &e,
                        ChebLaunch P, hipStream_t stream) {
-  const DeviceLimits &dl = device_limits();
-  // LDS: the weights (when they fit beside one wavefront's region) and one
-  // region per wavefront; as many wavefronts per workgroup as fit
-  const size_t wave_bytes = 4 * (size_t)wave_lds_words(P);
-  size_t wbytes = 4 * (size_t)((P.wsize + 3) / 4 * 4);
-  P.w_in_lds = P.wsize <= 16384 && wbytes + wave_bytes <= dl.lds_per_block;
-  if (!P.w_in_lds) wbytes = 0;
-  P.waves = (int)std::min<size_t>(MAX_WAVES, (dl.lds_per_block - wbytes) / wave_bytes);
-  if (P.waves < 1) {
+  PersistentShape ps;
+  const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
+  if (!persistent_shape(wbytes, 4 * (size_t)wave_lds_words(P), P.natoms, ps)) {
     // cannot happen within the capacity limits; refuse rather than overrun LDS
     raise_flag(b.flags, AENET_AMD_FLAG_NB_OVERFLOW, stream);
     return;
   }
-  const size_t lds = wbytes + (size_t)P.waves * wave_bytes;
-  // persistent grid: as many workgroups as fit on every CU at once
-  int per_cu = (int)(dl.lds_per_cu / lds);
-  per_cu = std::max(1, std::min(per_cu, 32 / P.waves));
-  int grid = std::min((P.natoms + P.waves - 1) / P.waves, dl.cus * per_cu);
-  grid = std::max(grid, 1);
-  hipLaunchKernelGGL((k_atom_cheb<RN, AN, MODE, F>), dim3(grid), dim3(64 * P.waves), lds, stream, md, b, nb, o, e,
-                     P);
+  P.w_in_lds = ps.w_in_lds;
+  P.waves = ps.waves;
+  hipLaunchKernelGGL((k_atom_cheb<RN, AN, MODE, F>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream, md, b, nb, o,
+                     e, P);
 }
 
 template <int RN, int AN>

@@ -1,0 +1,244 @@
+// atom_common.h -- pieces shared by the per-atom kernels (atom_cheb.hip,
+// atom_bp.hip): wavefront synchronisation, reduce-scatter of per-lane
+// partial sums, the per-species network (forward and dE/dG backward), the
+// per-wavefront LDS layout and the persistent-grid sizing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "aenet_amd_internal.h"
+#include "kernels.h"
+#include "kernels_common.h"
+
+namespace aenet_amd {
+
+constexpr int ATOM_MAX_WAVES = 4;  // wavefronts per workgroup (fewer when LDS is short)
+
+// orders the LDS accesses of one wavefront (lanes exchange data through
+// LDS between phases of one atom)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ void activate(int f, float x, float &y, float &dy) {
+  // ff_activate (src/ext/feedforward.f90:932-969)
+  switch (f) {
+    case 0: y = x; dy = 1.0f; break;
+    case 1: y = tanhf(x); dy = 1.0f - y * y; break;
+    case 2: y = 1.0f / (1.0f + expf(-x)); dy = y * (1.0f - y); break;
+    case 3: { float t = tanhf(0.666666666666667f * x); y = 1.7159f * t; dy = 1.7159f * (1.0f - t * t) * 0.666666666666667f; } break;
+    case 4: { float t = tanhf(0.666666666666667f * x); y = 1.7159f * t + 0.1f * x; dy = 1.7159f * (1.0f - t * t) * 0.666666666666667f + 0.1f; } break;
+    default: y = 0.0f; dy = 0.0f;
+  }
+}
+
+// cutoff function and its derivative: fc = (cos(pi r/Rc) + 1)/2 for
+// r < Rc, else 0 (sfb_fc / sfb_fc_d1, src/ext/sfbasis.f90:719-752; sf_cut,
+// src/ext/symmfunc.f90:823-838 is the same function)
+__device__ __forceinline__ void cutoff(float r, float Rc, float inv_Rc, float &fc, float &dfc) {
+  if (r >= Rc) {
+    fc = 0.0f;
+    dfc = 0.0f;
+  } else {
+    float sp, cp;
+    sincospif(r * inv_Rc, &sp, &cp);
+    fc = 0.5f * (cp + 1.0f);
+    dfc = -0.5f * 3.14159265358979f * inv_Rc * sp;
+  }
+}
+
+// triangular index p -> (a, b), 0 <= a < b < n, row-major
+__device__ __forceinline__ void tri_decode(int p, int n, int &a, int &b) {
+  float nn = (float)(2 * n - 1);
+  int r = (int)((nn - sqrtf(nn * nn - 8.0f * (float)p)) * 0.5f);
+  if (r < 0) r = 0;
+  if (r > n - 2) r = n - 2;
+  while (r > 0 && r * n - r * (r + 1) / 2 > p) --r;
+  while ((r + 1) * n - (r + 1) * (r + 2) / 2 <= p) ++r;
+  a = r;
+  b = p - (r * n - r * (r + 1) / 2) + r + 1;
+}
+
+// Reduce-scatter of V per-lane values (V a power of two <= 64) over the
+// wavefront: on return lane l holds the sum over all 64 lanes of
+// v[l >> (6 - log2 V)] (64/V consecutive lanes hold the same feature).
+template <int V>
+__device__ __forceinline__ double reduce_scatter(double (&v)[V]) {
+  static_assert(V >= 1 && V <= 64 && (V & (V - 1)) == 0, "power of two up to 64");
+  const int lane = lane_id();
+  int off = 32;
+#pragma unroll
+  for (int half = V / 2; half >= 1; half >>= 1) {
+    const bool up = lane & off;
+#pragma unroll
+    for (int k = 0; k < half; ++k) {
+      const double lo = v[k], hi = v[k + half];
+      const double send = up ? lo : hi;
+      const double keep = up ? hi : lo;
+      v[k] = keep + __shfl_xor(send, off, WAVE);
+    }
+    off >>= 1;
+  }
+  double s = v[0];
+#pragma unroll
+  for (int o = 32 / V; o >= 1; o >>= 1) s += __shfl_xor(s, o, WAVE);
+  return s;
+}
+
+// the same for FP32 partial sums: the first level in FP32, the rest FP64
+template <int V>
+__device__ __forceinline__ double reduce_scatter(float (&v)[V]) {
+  static_assert(V >= 2 && V <= 64 && (V & (V - 1)) == 0, "power of two 2..64");
+  const int lane = lane_id();
+  const bool up = lane & 32;
+  double d[V / 2];
+#pragma unroll
+  for (int k = 0; k < V / 2; ++k) {
+    const float lo = v[k], hi = v[k + V / 2];
+    const float send = up ? lo : hi;
+    const float keep = up ? hi : lo;
+    d[k] = (double)(keep + __shfl_xor(send, 32, WAVE));
+  }
+  int off = 16;
+#pragma unroll
+  for (int half = V / 4; half >= 1; half >>= 1) {
+    const bool u = lane & off;
+#pragma unroll
+    for (int k = 0; k < half; ++k) {
+      const double lo = d[k], hi = d[k + half];
+      const double send = u ? lo : hi;
+      const double keep = u ? hi : lo;
+      d[k] = keep + __shfl_xor(send, off, WAVE);
+    }
+    off >>= 1;
+  }
+  double s = d[0];
+#pragma unroll
+  for (int o = 32 / V; o >= 1; o >>= 1) s += __shfl_xor(s, o, WAVE);
+  return s;
+}
+
+// LDS of the network stage of one wavefront
+struct NetLds {
+  float *gp;    // [nsf] normalized descriptor
+  float *coef;  // [nsf] E_scale * dy/dG' * norm scale = dE/dG (unnormalized G)
+  float *d0;    // [dmax] back-propagation buffers, dmax = max(nsf, 64)
+  float *d1;    // [dmax]
+  float *act;   // [nl*64] outputs of layer l at act[64*l]
+  float *dact;  // [nl*64] activation derivatives
+};
+
+__host__ __device__ inline int net_dmax(int nsf) { return nsf > AENET_AMD_MAX_HIDDEN ? nsf : AENET_AMD_MAX_HIDDEN; }
+__host__ __device__ inline int net_lds_words(int nsf, int nl) {
+  return 2 * nsf + 2 * net_dmax(nsf) + 2 * nl * AENET_AMD_MAX_HIDDEN;
+}
+__device__ inline NetLds carve_net(float *p, int nsf, int nl) {
+  NetLds n;
+  n.gp = p;
+  n.coef = n.gp + nsf;
+  n.d0 = n.coef + nsf;
+  n.d1 = n.d0 + net_dmax(nsf);
+  n.act = n.d1 + net_dmax(nsf);
+  n.dact = n.act + nl * AENET_AMD_MAX_HIDDEN;
+  return n;
+}
+
+// Feed-forward network of one species on the normalized descriptor in
+// N.gp: ff_eval (src/ext/feedforward.f90:586-655), lanes = nodes of a
+// layer.  With grad, also ff_deriv (:661-757) and the chain through the
+// normalization and the energy scale: N.coef[f] = dE_i/dG_f.  Returns the
+// network output y (all lanes).
+__device__ inline float network(const ModelDev &md, const SpeciesDev &sp, const float *W, const NetLds &N,
+                                bool grad) {
+  const int lane = lane_id();
+  const int nl = sp.nlayers;
+  const float *in = N.gp;
+  for (int l = 0; l + 1 < nl; ++l) {
+    const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
+    const float *Wl = W + sp.w_off[l];
+    if (lane < n2) {
+      float z0 = Wl[n1 * n2 + lane], z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;  // bias column
+      int c = 0;
+      for (; c + 3 < n1; c += 4) {
+        z0 += Wl[c * n2 + lane] * in[c];
+        z1 += Wl[(c + 1) * n2 + lane] * in[c + 1];
+        z2 += Wl[(c + 2) * n2 + lane] * in[c + 2];
+        z3 += Wl[(c + 3) * n2 + lane] * in[c + 3];
+      }
+      for (; c < n1; ++c) z0 += Wl[c * n2 + lane] * in[c];
+      float y, dy;
+      activate(sp.fa[l + 1], (z0 + z1) + (z2 + z3), y, dy);
+      N.act[(l + 1) * AENET_AMD_MAX_HIDDEN + lane] = y;
+      N.dact[(l + 1) * AENET_AMD_MAX_HIDDEN + lane] = dy;
+    }
+    wave_sync();
+    in = N.act + (l + 1) * AENET_AMD_MAX_HIDDEN;
+  }
+  const float yout = N.act[(nl - 1) * AENET_AMD_MAX_HIDDEN];
+  if (!grad) return yout;
+  float *dcur = N.d0;
+  float *dnext = N.d1;
+  if (lane == 0) dcur[0] = N.dact[(nl - 1) * AENET_AMD_MAX_HIDDEN];
+  wave_sync();
+  for (int l = nl - 2; l >= 0; --l) {
+    const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
+    const float *Wl = W + sp.w_off[l];
+    for (int c = lane; c < n1; c += WAVE) {
+      const float *row = Wl + c * n2;
+      float s0 = 0.0f, s1 = 0.0f;
+      int hh = 0;
+      for (; hh + 1 < n2; hh += 2) {
+        s0 += row[hh] * dcur[hh];
+        s1 += row[hh + 1] * dcur[hh + 1];
+      }
+      if (hh < n2) s0 += row[hh] * dcur[hh];
+      float s = s0 + s1;
+      dnext[c] = l > 0 ? s * N.dact[l * AENET_AMD_MAX_HIDDEN + c] : s;
+    }
+    wave_sync();
+    float *t = dcur;
+    dcur = dnext;
+    dnext = t;
+  }
+  const float Es = (float)sp.E_scale;
+  for (int f = lane; f < sp.nsf; f += WAVE) N.coef[f] = Es * dcur[f] * (float)md.norm_scale[sp.norm_off + f];
+  wave_sync();
+  return yout;
+}
+
+// E_i = E_scale*y + E_shift + E_atom (src/aenet.f90:468-469), FP64
+__device__ __forceinline__ double atom_energy(const SpeciesDev &sp, float y) {
+  double E = sp.E_scale * (double)y + sp.E_shift;
+  return E + sp.E_atom;
+}
+
+// Persistent launch shape for an atom kernel whose workgroups hold wbytes
+// of shared data (the weights) plus wave_bytes per wavefront: as many
+// wavefronts per workgroup as LDS allows (up to ATOM_MAX_WAVES) and as many
+// workgroups as fit on the device at once.  Returns false when not even
+// one wavefront fits.
+struct PersistentShape {
+  int waves, grid;
+  size_t lds;
+  bool w_in_lds;
+};
+
+inline bool persistent_shape(size_t wbytes, size_t wave_bytes, int natoms, PersistentShape &ps) {
+  const DeviceLimits &dl = device_limits();
+  ps.w_in_lds = wbytes > 0 && wbytes + wave_bytes <= dl.lds_per_block && wbytes <= 64 * 1024;
+  if (!ps.w_in_lds) wbytes = 0;
+  if (wave_bytes > dl.lds_per_block - wbytes) return false;
+  ps.waves = (int)std::min<size_t>(ATOM_MAX_WAVES, (dl.lds_per_block - wbytes) / wave_bytes);
+  ps.lds = wbytes + (size_t)ps.waves * wave_bytes;
+  int per_cu = (int)(dl.lds_per_cu / ps.lds);
+  per_cu = std::max(1, std::min(per_cu, 32 / ps.waves));
+  ps.grid = std::min((natoms + ps.waves - 1) / ps.waves, dl.cus * per_cu);
+  ps.grid = std::max(ps.grid, 1);
+  return true;
+}
+
+}  // namespace aenet_amd

@@ -92,11 +92,13 @@ def test_neighbor_lists_bit_exact(path, device):
         ann.release()
 
 
-def test_per_atom_api_matches_reference(device):
+@pytest.mark.parametrize("case", ["ref_01_alloy", "ref_03_bp3_fcc"])
+def test_per_atom_api_matches_reference(case, device):
     """aenet_atomic_energy_and_forces on the reference's own neighbor lists
-    (the per-atom drop-in entry point, src/aenet.f90:410-486)."""
+    (the per-atom drop-in entry point, src/aenet.f90:410-486), Chebyshev
+    and Behler-Parrinello."""
     from aenet_amd.core import ANNPotentials
-    path = [p for p in FIXTURES if "alloy" in p][0]
+    path = [p for p in FIXTURES if os.path.basename(p).startswith(case)][0]
     g = _load(path)
     ann = ANNPotentials(_pots(g))
     try:
@@ -163,3 +165,35 @@ def test_sfb_eval_matches_oracle(device):
         assert np.allclose(vals, ref, rtol=2e-6, atol=2e-5)
     finally:
         L.aenet_sfb_final(C.byref(stat))
+
+
+@pytest.mark.parametrize("case", ["ref_04_bp3_slab", "ref_05_bp3_cluster"])
+def test_bp_descriptors_match_oracle(case, device):
+    """Unscaled Behler-Parrinello descriptors (G1-G5 of sf_fingerprint,
+    src/ext/symmfunc.f90:311-431) of every atom against the FP64 oracle on
+    the reference's neighbor lists."""
+    from aenet_amd.core import ANNPotentials
+    from aenet_amd.device import DeviceBatch
+    from oracle_lib import Oracle
+    path = [p for p in FIXTURES if os.path.basename(p).startswith(case)][0]
+    g = _load(path)
+    names = [str(t) for t in g["types"]]
+    ann = ANNPotentials(_pots(g))
+    try:
+        db = DeviceBatch([_S(g)], device=device)
+        nsf = 132
+        G = db.descriptors(nsf, scaled=False).cpu().numpy()
+        del db
+    finally:
+        ann.release()
+    o = Oracle(names, _pots(g))
+    L = g["latt"].T
+    worst = 0.0
+    for i in range(len(g["atom_types"])):
+        k = g["nnb"][i]
+        ref = o.descriptors(int(g["atom_types"][i]), L @ g["frac"][i], g["nb_coo"][i, :k], g["nb_type"][i, :k],
+                            scaled=False)
+        err = np.abs(G[i, :len(ref)] - ref) / (1.0 + np.abs(ref))
+        worst = max(worst, float(err.max()))
+    print("max relative |dG| %.3e" % worst)
+    assert worst < 1e-5
