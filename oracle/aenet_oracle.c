@@ -7,7 +7,7 @@
  * that the FP64 neighbor arithmetic is evaluated term by term, in the order
  * the reference Fortran writes it.
  *
- * Parity pins (see tests/test_oracle_*.py): the reference library built from
+ * Parity pins (tests/test_oracle.py): the reference library built from
  * its own sources (oracle/_ref), the golden energy of
  * python3/aenet/tests/test_ase.py:58, FCC neighbor counts of
  * src/tests/test_lclist.f90, closed forms and finite differences.
