@@ -127,6 +127,87 @@ def pair_triplet_counts(db, rc_r=6.5, rc_a=4.5):
     return int(n_p.sum().item()), int(n_t.sum().item())
 
 
+# Behler-Parrinello batch (2048 structures of 8-512 atoms, 3 species, 8 G2
+# per neighbor species and 18 G4 per species pair at Rc = 6 A, 132-25-25-1):
+# per pair 24 + 14*n_G2, per contributing triplet (all legs < Rc)
+# 40 + 16*n_G4, network 4*Wsize + 3*nsf per atom
+BP_FLOP_PAIR = 24 + 14 * 8
+BP_FLOP_TRIPLET = 40 + 16 * 18
+BP_WSIZE = 133 * 25 + 26 * 25 + 26
+BP_FLOP_ATOM = 4 * BP_WSIZE + 3 * 132
+
+
+def bp_counts(db, rc=6.0, chunk=2048):
+    """Pairs with r < rc and triplets with all three legs < rc, from the
+    batch's own neighbor lists (counted on the GPU)."""
+    import torch
+    nb = db.neighbors(160)
+    n = nb["nnb"].long()
+    pairs = triplets = 0
+    for a0 in range(0, db.natoms, chunk):
+        coo = nb["coo"][a0:a0 + chunk]
+        d = nb["dist"][a0:a0 + chunk]
+        cnt = n[a0:a0 + chunk]
+        valid = torch.arange(d.shape[1], device=d.device)[None, :] < cnt[:, None]
+        inside = valid & (d < rc)
+        pairs += int(inside.sum().item())
+        djk = torch.cdist(coo, coo)
+        m = inside[:, :, None] & inside[:, None, :] & (djk < rc)
+        triplets += int(torch.triu(m.to(torch.int8), diagonal=1).sum().item())
+    return pairs, triplets
+
+
+def batched_bp(steps, warmup):
+    """Throughput and atom-kernel roofline of the batched BP workload."""
+    import torch
+    from aenet_amd import synth
+    from aenet_amd.core import ANNPotentials
+    from aenet_amd.device import DeviceBatch
+    d = os.path.join(ROOT, "tests", "golden", "potentials")
+    ann = ANNPotentials({t: os.path.join(d, "bp3.%s.nn" % t) for t in "ABC"})
+    try:
+        structs = synth.batch_uneven(2048)
+        db = DeviceBatch(structs)
+        for _ in range(warmup):
+            db.eval(check=False)
+        db.check()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            db.eval(check=False)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        db.check()
+        ms = (t1 - t0) * 1e3 / steps
+        db.set_timing(True)
+        for _ in range(steps):
+            db.eval(check=False)
+        kms, kcount = db.kernel_time()
+        db.set_timing(False)
+        k_s = kms / kcount * 1e-3
+        n_p, n_t = bp_counts(db)
+        flops = n_p * BP_FLOP_PAIR + n_t * BP_FLOP_TRIPLET + db.natoms * BP_FLOP_ATOM
+        out = {
+            "workload": "batch2048: 2048 structures, 8-512 atoms (log-uniform), bulk FCC/BCC, slabs, clusters; "
+                        "3 species; BP 24 G2 + 108 G4 at Rc 6 A; 132-25-25-1",
+            "structures": db.nstruct,
+            "atoms_per_step": db.natoms,
+            "ms_per_step": ms,
+            "atoms_per_s": db.natoms / (ms * 1e-3),
+            "kernel": "k_atom_bp<0,true>",
+            "kernel_ms": k_s * 1e3,
+            "pairs": n_p,
+            "contributing_triplets": n_t,
+            "flop_per_launch": flops,
+            "achieved_tflops": flops / k_s / 1e12,
+            "frac_fp32_peak": flops / k_s / FP32_PEAK,
+        }
+        del db
+        return out
+    finally:
+        ann.release()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +218,7 @@ def main():
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU baseline worker processes (0: host cores, <=16)")
     ap.add_argument("--cpu-stride", type=int, default=1, help="sample = every (procs*stride)th atom")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched Behler-Parrinello measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the prediction path runs on a single MI355X and this build "
@@ -232,6 +314,8 @@ def main():
         out["gpu_over_cpu"] = value / cpu["value"]
         out["gpu_over_cpu_end_to_end"] = value / cpu["end_to_end"]
     ann.release()
+    if not args.no_batch:
+        out["batched_bp"] = batched_bp(max(5, min(args.steps, 20)), 2)
     print(json.dumps(out))
 
 
