@@ -25,6 +25,57 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _pint = C.POINTER(C.c_int)
 _vp = C.c_void_p
 
+# C signatures of every function of include/aenet.h and include/aenet_amd.h
+SIGNATURES = {
+    "aenet_init": (None, [C.c_int, C.POINTER(C.c_char_p), _pint]),
+    "aenet_final": (None, [_pint]),
+    "aenet_print_info": (None, []),
+    "aenet_load_potential": (None, [C.c_int, C.c_char_p, _pint]),
+    "aenet_all_loaded": (C.c_bool, []),
+    "aenet_free_atom_energy": (C.c_double, [C.c_int]),
+    "aenet_atomic_energy": (None, [_dp, C.c_int, C.c_int, _dp, _ip, _dp, _pint]),
+    "aenet_atomic_energy_and_forces": (None, [_dp, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, C.c_int, _dp,
+                                              _dp, _pint]),
+    "aenet_convert_atom_types": (None, [C.c_int, C.POINTER(C.c_char_p), C.c_int, _ip, _ip, _pint]),
+    "aenet_sfb_init": (None, [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_int, C.c_double, C.c_double,
+                              _pint]),
+    "aenet_sfb_final": (None, [_pint]),
+    "aenet_sfb_nvalues": (C.c_int, []),
+    "aenet_sfb_eval": (None, [C.c_int, _dp, C.c_int, _ip, _dp, C.c_int, _dp, _pint]),
+    "aenet_sfb_reconstruct_radial": (None, [C.c_int, _dp, C.c_int, _dp, _dp, _pint]),
+    "aenet_nbl_init": (None, [_dp, C.c_int, _ip, _dp, C.c_bool, C.c_bool]),
+    "aenet_nbl_final": (None, []),
+    "aenet_nbl_neighbors": (None, [C.c_int, _pint, _dp, _dp, _ip, _ip]),
+    "aenet_structures_energy_and_forces": (None, [C.c_int, _ip, _dp, _ip, _ip, _dp, _vp, _vp, _vp, _vp,
+                                                  _pint]),
+    "aenet_amd_batch_create": (None, [C.c_int, _ip, _dp, _ip, _dp, C.POINTER(_vp), _pint]),
+    "aenet_amd_batch_destroy": (None, [_vp]),
+    "aenet_amd_batch_natoms": (C.c_int, [_vp]),
+    "aenet_amd_batch_eval": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
+    "aenet_amd_batch_check": (None, [_vp, _vp, _pint]),
+    "aenet_amd_batch_set_timing": (None, [_vp, C.c_int]),
+    "aenet_amd_batch_kernel_time": (None, [_vp, C.POINTER(C.c_double), _pint, _pint]),
+    "aenet_amd_batch_neighbors": (None, [_vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
+    "aenet_amd_batch_descriptors": (None, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _pint]),
+    "aenet_amd_model_load": (None, [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.POINTER(_vp),
+                                    _pint]),
+    "aenet_amd_model_destroy": (None, [_vp]),
+    "aenet_amd_model_info": (None, [_vp, C.POINTER(C.c_double), _pint, _pint, _pint]),
+    "aenet_amd_model_species_weights": (None, [_vp, C.c_int, _pint, _pint, _pint]),
+    "aenet_amd_model_weights": (None, [_vp, _vp, C.c_int, _vp, _pint]),
+    "aenet_amd_batch_create_model": (None, [_vp, C.c_int, _ip, _dp, _ip, _dp, C.POINTER(_vp), _pint]),
+    "aenet_amd_batch_eval_model": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
+    "aenet_amd_batch_eval_active": (None, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _pint]),
+    "aenet_amd_batch_vjp": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
+    "aenet_amd_batch_descriptors_model": (None, [_vp, _vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _pint]),
+    "aenet_amd_model_net_grad": (None, [_vp, C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, C.c_double, _vp, _vp,
+                                        _vp, _vp, _pint]),
+    "aenet_amd_model_adam": (None, [_vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                    C.c_double, _vp, _pint]),
+    "aenet_amd_model_check": (None, [_vp, _vp, _pint]),
+    "aenet_amd_last_error": (C.c_char_p, []),
+}
+
 _lib = None
 
 
@@ -41,40 +92,7 @@ def lib():
         raise NativeLibraryMissing(
             "libaenet_amd.so not found at %s; build it with `python -m aenet_amd.build`" % LIB_PATH)
     L = C.CDLL(LIB_PATH)
-    sig = {
-        "aenet_init": (None, [C.c_int, C.POINTER(C.c_char_p), _pint]),
-        "aenet_final": (None, [_pint]),
-        "aenet_print_info": (None, []),
-        "aenet_load_potential": (None, [C.c_int, C.c_char_p, _pint]),
-        "aenet_all_loaded": (C.c_bool, []),
-        "aenet_free_atom_energy": (C.c_double, [C.c_int]),
-        "aenet_atomic_energy": (None, [_dp, C.c_int, C.c_int, _dp, _ip, _dp, _pint]),
-        "aenet_atomic_energy_and_forces": (None, [_dp, C.c_int, C.c_int, C.c_int, _dp, _ip, _ip, C.c_int, _dp,
-                                                  _dp, _pint]),
-        "aenet_convert_atom_types": (None, [C.c_int, C.POINTER(C.c_char_p), C.c_int, _ip, _ip, _pint]),
-        "aenet_sfb_init": (None, [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.c_int, C.c_double, C.c_double,
-                                  _pint]),
-        "aenet_sfb_final": (None, [_pint]),
-        "aenet_sfb_nvalues": (C.c_int, []),
-        "aenet_sfb_eval": (None, [C.c_int, _dp, C.c_int, _ip, _dp, C.c_int, _dp, _pint]),
-        "aenet_sfb_reconstruct_radial": (None, [C.c_int, _dp, C.c_int, _dp, _dp, _pint]),
-        "aenet_nbl_init": (None, [_dp, C.c_int, _ip, _dp, C.c_bool, C.c_bool]),
-        "aenet_nbl_final": (None, []),
-        "aenet_nbl_neighbors": (None, [C.c_int, _pint, _dp, _dp, _ip, _ip]),
-        "aenet_structures_energy_and_forces": (None, [C.c_int, _ip, _dp, _ip, _ip, _dp, _vp, _vp, _vp, _vp,
-                                                      _pint]),
-        "aenet_amd_batch_create": (None, [C.c_int, _ip, _dp, _ip, _dp, C.POINTER(_vp), _pint]),
-        "aenet_amd_batch_destroy": (None, [_vp]),
-        "aenet_amd_batch_natoms": (C.c_int, [_vp]),
-        "aenet_amd_batch_eval": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
-        "aenet_amd_batch_check": (None, [_vp, _vp, _pint]),
-        "aenet_amd_batch_set_timing": (None, [_vp, C.c_int]),
-        "aenet_amd_batch_kernel_time": (None, [_vp, C.POINTER(C.c_double), _pint, _pint]),
-        "aenet_amd_batch_neighbors": (None, [_vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
-        "aenet_amd_batch_descriptors": (None, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _pint]),
-        "aenet_amd_last_error": (C.c_char_p, []),
-    }
-    for name, (res, args) in sig.items():
+    for name, (res, args) in SIGNATURES.items():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
@@ -97,15 +115,7 @@ def global_double(name):
 
 def exported_symbols():
     """Names declared in include/aenet.h and include/aenet_amd.h."""
-    funcs = ["aenet_init", "aenet_final", "aenet_print_info", "aenet_load_potential", "aenet_all_loaded",
-             "aenet_free_atom_energy", "aenet_atomic_energy", "aenet_atomic_energy_and_forces",
-             "aenet_convert_atom_types", "aenet_sfb_init", "aenet_sfb_final", "aenet_sfb_nvalues",
-             "aenet_sfb_eval", "aenet_sfb_reconstruct_radial", "aenet_nbl_init", "aenet_nbl_final",
-             "aenet_nbl_neighbors", "aenet_structures_energy_and_forces", "aenet_amd_batch_create",
-             "aenet_amd_batch_destroy", "aenet_amd_batch_natoms", "aenet_amd_batch_eval",
-             "aenet_amd_batch_check", "aenet_amd_batch_set_timing", "aenet_amd_batch_kernel_time",
-             "aenet_amd_batch_neighbors", "aenet_amd_batch_descriptors",
-             "aenet_amd_last_error"]
+    funcs = list(SIGNATURES)
     data = ["AENET_OK", "AENET_ERR_INIT", "AENET_ERR_MALLOC", "AENET_ERR_IO", "AENET_ERR_TYPE",
             "AENET_TYPELEN", "AENET_PATHLEN", "AENET_TRUE", "AENET_FALSE", "aenet_nsf_max", "aenet_nnb_max",
             "aenet_Rc_min", "aenet_Rc_max"]

@@ -89,6 +89,26 @@ struct AtomSizes {
   int nsf_max;  // longest descriptor
   int wsize;    // total network weights
   int nl_max;   // most network layers (input layer included)
+  // optional subset of central atoms (structures mode): atom indices
+  // active[0 .. *nactive-1]; natoms is then the largest possible count
+  const int *active;
+  const int *nactive;
+};
+
+// network training (train.hip): per-atom normalized descriptors in, atomic
+// energies and the weight gradient of sum_i g_i E_i out
+struct NetGradArgs {
+  const double *G;         // [natoms*stride] normalized descriptors G'
+  int stride;
+  const int *types;        // [natoms] 1-based species
+  const double *E_ref;     // optional targets: g_i = coef * weight_i * (E_i - E_ref_i)
+  const double *weight;    // optional per-atom weights (with E_ref)
+  const double *upstream;  // g_i given directly (when E_ref is null)
+  double coef;
+  double *E_out;           // optional [natoms] predicted atomic energies
+  float *grad;             // [wsize] gradient, accumulated
+  double *loss;            // optional: += sum_i weight_i (E_i - E_ref_i)^2
+  int *flags;              // optional error word
 };
 
 struct ModelDev {
@@ -146,6 +166,7 @@ struct OutDev {
   int G_scaled;
   int G_stride;
   int desc_only;       // stop after the descriptors
+  const double *g_atom;  // optional [natoms]: forces of sum_i g_i E_i instead of sum_i E_i
   // neighbor-list mode (bit-exact lists)
   int nb_cap;
   int *nb_count;

@@ -67,6 +67,8 @@ struct BPLaunch {
   int wsize;     // total weights
   int w_in_lds;  // weights staged in LDS
   int waves;     // wavefronts per workgroup
+  const int *active;   // optional subset of central atoms (mode 0)
+  const int *nactive;
 };
 
 __host__ __device__ inline int bp_wave_words(const BPLaunch &P) {
@@ -343,7 +345,9 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
   const BPDev &bp = *md.bp;
   const int nt = bp.ntypes;
 
-  for (int g = blockIdx.x * nw + wave; g < P.natoms; g += gridDim.x * nw) {
+  const int count = P.active ? *P.nactive : P.natoms;
+  for (int k = blockIdx.x * nw + wave; k < count; k += gridDim.x * nw) {
+    const int g = P.active ? P.active[k] : k;
     wave_sync();
     // ------------------------------------------------------------ 1
     int ti, n;
@@ -505,6 +509,12 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
         env.E[g] = atom_energy(sp, yout);
     }
     if (!FORCES) continue;
+    if (MODE == 0 && o.g_atom) {
+      // forces of g_i * E_i: scale dE_i/dG
+      const float gs = (float)o.g_atom[g];
+      for (int f = lane; f < sp.nsf; f += WAVE) S.net.coef[f] *= gs;
+      wave_sync();
+    }
 
     // ------------------------------------------------------------ 5
     const float *coef = S.net.coef;
@@ -614,6 +624,8 @@ void launch_atom_bp(int mode, bool forces, const AtomSizes &z, const ModelDev &m
   P.wsize = z.wsize;
   P.w_in_lds = 0;
   P.waves = 1;
+  P.active = mode == 0 ? z.active : nullptr;
+  P.nactive = mode == 0 ? z.nactive : nullptr;
   if (mode == 0) {
     if (forces)
       launch_bp<0, true>(md, b, nb, o, e, P, stream);

@@ -52,6 +52,8 @@ struct ChebLaunch {
   int wsize;     // total weights
   int w_in_lds;  // weights staged in LDS
   int waves;     // wavefronts per workgroup
+  const int *active;   // optional subset of central atoms (mode 0)
+  const int *nactive;
 };
 
 __host__ __device__ inline int wave_lds_words(const ChebLaunch &P) { return 10 * P.cap + net_lds_words(P.nsf, P.nl); }
@@ -89,7 +91,9 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_cheb(ModelDev md, 
   const int wsz = P.w_in_lds ? (P.wsize + 3) / 4 * 4 : 0;
   WaveLds S = carve(lds + wsz + wave * wave_lds_words(P), P);
 
-  for (int g = blockIdx.x * nw + wave; g < P.natoms; g += gridDim.x * nw) {
+  const int count = P.active ? *P.nactive : P.natoms;
+  for (int k = blockIdx.x * nw + wave; k < count; k += gridDim.x * nw) {
+    const int g = P.active ? P.active[k] : k;
     wave_sync();
     // ------------------------------------------------------------ 1
     int ti, n;
@@ -239,6 +243,12 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_cheb(ModelDev md, 
         env.E[g] = atom_energy(sp, yout);
     }
     if (!FORCES) continue;
+    if (MODE == 0 && o.g_atom) {
+      // forces of g_i * E_i: scale dE_i/dG
+      const float gs = (float)o.g_atom[g];
+      for (int f = lane; f < sp.nsf; f += WAVE) S.net.coef[f] *= gs;
+      wave_sync();
+    }
 
     // ------------------------------------------------------------ 5
     float *fs = S.fs;
@@ -405,6 +415,8 @@ void launch_atom_cheb(int rN, int aN, int mode, bool forces, const AtomSizes &z,
   P.wsize = z.wsize;
   P.w_in_lds = 0;
   P.waves = 1;
+  P.active = mode == 0 ? z.active : nullptr;
+  P.nactive = mode == 0 ? z.nactive : nullptr;
   if (rN <= 9 && aN <= 1)
     launch_cheb<9, 1>(mode, forces, md, b, nb, o, e, P, stream);
   else if (rN <= 17 && aN <= 5)

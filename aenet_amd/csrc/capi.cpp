@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "aenet_capi_status.h"
+#include "kernels.h"
 #include "engine.h"
 #include "model.h"
 
@@ -131,6 +132,22 @@ int eval_env(Global &g, const double *coo_i, int type_i, int n_j, const double *
 struct aenet_amd_batch {
   Batch batch;
 };
+
+struct aenet_amd_model {
+  PotSet set;
+  DeviceModel dev;
+  DBuf<int> flags;  // error word of the training kernels
+};
+
+namespace {
+// the model an entry point works with: the given handle, else the set
+// loaded through aenet_load_potential
+const DeviceModel *pick_model(const aenet_amd_model *m) {
+  if (m) return &m->dev;
+  Global &g = G();
+  return g.dev ? g.dev.get() : nullptr;
+}
+}  // namespace
 
 extern "C" {
 
@@ -708,7 +725,7 @@ void aenet_amd_batch_eval(aenet_amd_batch *batch, const double *d_coords, const 
     *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or potentials not loaded");
     return;
   }
-  hipStream_t s = stream ? (hipStream_t)stream : lib_stream();
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
   *stat = batch->batch.eval(*g.dev, d_coords, d_types, d_E_atoms, d_E_struct, d_F, s);
   if (*stat) set_err(*stat, "kernel launch failed");
 }
@@ -732,7 +749,7 @@ void aenet_amd_batch_check(aenet_amd_batch *batch, void *stream, int *stat) {
     return;
   }
   std::string err;
-  hipStream_t s = stream ? (hipStream_t)stream : lib_stream();
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
   *stat = batch->batch.check_flags(s, err);
   if (*stat) set_err(*stat, err);
 }
@@ -744,7 +761,7 @@ void aenet_amd_batch_neighbors(aenet_amd_batch *batch, const double *d_coords, c
     *stat = set_err(AENET_STATUS_ERR_INIT, "no batch");
     return;
   }
-  hipStream_t s = stream ? (hipStream_t)stream : lib_stream();
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
   *stat = batch->batch.neighbor_lists(d_coords, d_types, cap, d_count, d_coo, d_dist, d_list, d_type, s);
   if (*stat) set_err(*stat, "kernel launch failed");
 }
@@ -760,9 +777,251 @@ void aenet_amd_batch_descriptors(aenet_amd_batch *batch, const double *d_coords,
     *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "descriptor stride smaller than nsf_max");
     return;
   }
-  hipStream_t s = stream ? (hipStream_t)stream : lib_stream();
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
   *stat = batch->batch.descriptors(*g.dev, d_coords, d_types, scaled, d_G, stride, s);
   if (*stat) set_err(*stat, "kernel launch failed");
+}
+
+
+// ---------------------------------------------------------------------
+// independent potential sets (include/aenet_amd.h)
+// ---------------------------------------------------------------------
+
+void aenet_amd_model_load(int ntypes, char *atom_types[], char *files[], aenet_amd_model **model, int *stat) {
+  *model = nullptr;
+  if (ntypes < 1 || ntypes > AENET_AMD_MAX_TYPES) {
+    *stat = set_err(AENET_STATUS_ERR_MALLOC, "aenet_amd_model_load: 1 to 8 species are supported");
+    return;
+  }
+  if ((*stat = ensure_device())) return;
+  std::unique_ptr<aenet_amd_model> m(new aenet_amd_model());
+  m->set.ntypes = ntypes;
+  for (int i = 0; i < ntypes; ++i) m->set.names.push_back(cname(atom_types[i]));
+  m->set.pots.assign(ntypes, PotHost());
+  std::string err;
+  for (int t = 0; t < ntypes; ++t) {
+    struct stat sb;
+    if (!files[t] || ::stat(files[t], &sb) != 0) {
+      *stat = set_err(AENET_STATUS_ERR_IO, std::string("aenet_amd_model_load: file not found: ") +
+                                               (files[t] ? files[t] : "(null)"));
+      return;
+    }
+    int st = read_nn(files[t], m->set.pots[t], err);
+    if (st) {
+      *stat = set_err(st, err);
+      return;
+    }
+  }
+  int st = m->set.finalize(err);
+  if (st) {
+    *stat = set_err(st, err);
+    return;
+  }
+  hipStream_t s = lib_stream();
+  if (m->dev.upload(m->set, s) || m->flags.alloc(1)) {
+    *stat = set_err(AENET_STATUS_ERR_DEVICE, "aenet_amd_model_load: device upload failed");
+    return;
+  }
+  launch_fill_i32(m->flags.p, 0, 1, s);
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    *stat = set_err(AENET_STATUS_ERR_DEVICE, "aenet_amd_model_load: device error");
+    return;
+  }
+  *model = m.release();
+  *stat = AENET_STATUS_OK;
+}
+
+void aenet_amd_model_destroy(aenet_amd_model *model) { delete model; }
+
+void aenet_amd_model_info(const aenet_amd_model *model, double *Rc_max, int *nsf_max, int *wsize, int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!dm) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no model");
+    return;
+  }
+  if (Rc_max) *Rc_max = dm->rc_max;
+  if (nsf_max) *nsf_max = dm->nsf_max;
+  if (wsize) *wsize = dm->wsize;
+  *stat = AENET_STATUS_OK;
+}
+
+void aenet_amd_model_species_weights(const aenet_amd_model *model, int type_id, int *offset, int *size,
+                                     int *stat) {
+  const PotSet *ps = model ? &model->set : (G().dev ? &G().set : nullptr);
+  if (!ps || type_id < 1 || type_id > ps->ntypes) {
+    *stat = set_err(AENET_STATUS_ERR_TYPE, "no model or type id out of range");
+    return;
+  }
+  const SpeciesDev &sp = ps->species[type_id - 1];
+  *offset = sp.w_off[0];
+  *size = ps->pots[type_id - 1].net.Wsize;
+  *stat = AENET_STATUS_OK;
+}
+
+void aenet_amd_model_weights(aenet_amd_model *model, float *d_W, int to_model, void *stream, int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!dm || !d_W) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no model or no buffer");
+    return;
+  }
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
+  float *W = const_cast<float *>(dm->W.p);
+  hipError_t e = to_model ? hipMemcpyAsync(W, d_W, dm->wsize * sizeof(float), hipMemcpyDeviceToDevice, s)
+                          : hipMemcpyAsync(d_W, W, dm->wsize * sizeof(float), hipMemcpyDeviceToDevice, s);
+  *stat = e == hipSuccess ? AENET_STATUS_OK : set_err(AENET_STATUS_ERR_DEVICE, "weight copy failed");
+}
+
+void aenet_amd_batch_create_model(const aenet_amd_model *model, int nstruct, const int natoms[], const double latt[],
+                                  const int pbc[], const double coords_host[], aenet_amd_batch **batch, int *stat) {
+  if (!model) {
+    aenet_amd_batch_create(nstruct, natoms, latt, pbc, coords_host, batch, stat);
+    return;
+  }
+  *batch = nullptr;
+  std::unique_ptr<aenet_amd_batch> b(new aenet_amd_batch());
+  std::string err;
+  int st = b->batch.create(model->set, nstruct, natoms, latt, pbc, coords_host, err);
+  if (st) {
+    *stat = set_err(st, err);
+    return;
+  }
+  *batch = b.release();
+  *stat = AENET_STATUS_OK;
+}
+
+void aenet_amd_batch_eval_model(aenet_amd_batch *batch, const aenet_amd_model *model, const double *d_coords,
+                                const int *d_types, double *d_E_atoms, double *d_E_struct, double *d_F, void *stream,
+                                int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!batch || !dm) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or no model");
+    return;
+  }
+  if (!batch->batch.fits(*dm)) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "model cutoff exceeds the cutoff the batch was created for");
+    return;
+  }
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
+  *stat = batch->batch.eval(*dm, d_coords, d_types, d_E_atoms, d_E_struct, d_F, s);
+  if (*stat) set_err(*stat, "kernel launch failed");
+}
+
+void aenet_amd_batch_eval_active(aenet_amd_batch *batch, const aenet_amd_model *model, const double *d_coords,
+                                 const int *d_types, const int *d_active, const int *d_nactive, int max_active,
+                                 int reuse_lists, double *d_E_atoms, void *stream, int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!batch || !dm) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or no model");
+    return;
+  }
+  if (!d_active || !d_nactive || !d_E_atoms || max_active < 0 || max_active > batch->batch.natoms()) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_amd_batch_eval_active: bad arguments");
+    return;
+  }
+  if (!batch->batch.fits(*dm)) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "model cutoff exceeds the cutoff the batch was created for");
+    return;
+  }
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
+  *stat = batch->batch.eval_active(*dm, d_coords, d_types, d_active, d_nactive, max_active, reuse_lists != 0,
+                                   d_E_atoms, s);
+  if (*stat) set_err(*stat, "kernel launch failed");
+}
+
+void aenet_amd_batch_vjp(aenet_amd_batch *batch, const aenet_amd_model *model, const double *d_coords,
+                         const int *d_types, const double *d_g, double *d_grad_coords, float *d_grad_W, void *stream,
+                         int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!batch || !dm) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or no model");
+    return;
+  }
+  if (!d_g || !d_grad_coords || !batch->batch.fits(*dm)) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_amd_batch_vjp: bad arguments or model cutoff too large");
+    return;
+  }
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
+  *stat = batch->batch.vjp(*dm, d_coords, d_types, d_g, d_grad_coords, d_grad_W, s);
+  if (*stat) set_err(*stat, "kernel launch failed");
+}
+
+void aenet_amd_batch_descriptors_model(aenet_amd_batch *batch, const aenet_amd_model *model, const double *d_coords,
+                                       const int *d_types, int scaled, double *d_G, int stride, void *stream,
+                                       int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!batch || !dm) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or no model");
+    return;
+  }
+  if (stride < dm->nsf_max || !batch->batch.fits(*dm)) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "descriptor stride smaller than nsf_max, or model cutoff too large");
+    return;
+  }
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
+  *stat = batch->batch.descriptors(*dm, d_coords, d_types, scaled, d_G, stride, s);
+  if (*stat) set_err(*stat, "kernel launch failed");
+}
+
+void aenet_amd_model_net_grad(aenet_amd_model *model, int natoms, const double *d_G, int stride, const int *d_types,
+                              const double *d_E_ref, const double *d_weight, const double *d_upstream, double coef,
+                              double *d_E_out, float *d_grad, double *d_loss, void *stream, int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!dm) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no model");
+    return;
+  }
+  if (natoms < 0 || !d_G || !d_types || stride < dm->nsf_max || (!d_E_ref && !d_upstream && d_grad)) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_amd_model_net_grad: bad arguments");
+    return;
+  }
+  NetGradArgs a;
+  a.G = d_G;
+  a.stride = stride;
+  a.types = d_types;
+  a.E_ref = d_E_ref;
+  a.weight = d_weight;
+  a.upstream = d_upstream;
+  a.coef = coef;
+  a.E_out = d_E_out;
+  a.grad = d_grad;
+  a.loss = d_loss;
+  a.flags = model ? model->flags.p : nullptr;
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
+  launch_net_grad(dm->view(), a, natoms, dm->nsf_max, dm->nl_max, dm->wsize, d_grad != nullptr, s);
+  *stat = hipGetLastError() == hipSuccess ? AENET_STATUS_OK : set_err(AENET_STATUS_ERR_DEVICE, "launch failed");
+}
+
+void aenet_amd_model_adam(aenet_amd_model *model, const float *d_grad, float *d_m, float *d_v, int t, double lr,
+                          double b1, double b2, double eps, void *stream, int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!dm || !d_grad || !d_m || !d_v || t < 1) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_amd_model_adam: bad arguments");
+    return;
+  }
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
+  launch_adam(const_cast<float *>(dm->W.p), d_grad, d_m, d_v, dm->wsize, t, (float)lr, (float)b1, (float)b2,
+              (float)eps, s);
+  *stat = hipGetLastError() == hipSuccess ? AENET_STATUS_OK : set_err(AENET_STATUS_ERR_DEVICE, "launch failed");
+}
+
+void aenet_amd_model_check(aenet_amd_model *model, void *stream, int *stat) {
+  if (!model) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no model");
+    return;
+  }
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
+  int h = 0;
+  if (hipMemcpyAsync(&h, model->flags.p, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    *stat = set_err(AENET_STATUS_ERR_DEVICE, "device error");
+    return;
+  }
+  if (h) {
+    launch_fill_i32(model->flags.p, 0, 1, s);
+    *stat = set_err(AENET_STATUS_ERR_TYPE, "training kernel reported an atom type outside the model");
+    return;
+  }
+  *stat = AENET_STATUS_OK;
 }
 
 }  // extern "C"

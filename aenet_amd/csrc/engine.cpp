@@ -17,6 +17,7 @@ int DeviceModel::upload(const PotSet &ps, hipStream_t s) {
   max_aN = ps.max_aN;
   nsf_max = ps.nsf_max;
   wsize = (int)ps.W.size();
+  rc_max = ps.Rc_max;
   nl_max = 0;
   for (const SpeciesDev &sd : ps.species) nl_max = std::max(nl_max, sd.nlayers);
   if (species.upload(ps.species.data(), ps.species.size(), s)) return AENET_STATUS_ERR_DEVICE;
@@ -44,12 +45,14 @@ ModelDev DeviceModel::view() const {
 
 int Batch::create(const PotSet &ps, int nstruct, const int *natoms, const double *latt, const int *pbc,
                   const double *coords, std::string &err) {
+  rc_max_ = ps.Rc_max;
   int st = build_plan(ps.Rc_max, nstruct, natoms, latt, pbc, coords, plan_, err);
   if (st) return st;
   return allocate(err);
 }
 
 int Batch::create_lattice_coords(const PotSet &ps, int natoms, const double *latt, int pbc, std::string &err) {
+  rc_max_ = ps.Rc_max;
   int st = build_plan_lattice(ps.Rc_max, natoms, latt, pbc, plan_, err);
   if (st) return st;
   return allocate(err);
@@ -197,6 +200,78 @@ int Batch::eval(const DeviceModel &dm, const double *d_coords, const int *d_type
     // E_struct = [E_tot(0..ns-1), E_coh(0..ns-1)]
     launch_struct_energy(b, md, o.E_atom, d_E_struct, d_E_struct + b.nstruct, stream);
   }
+  return hipGetLastError() == hipSuccess ? AENET_STATUS_OK : AENET_STATUS_ERR_DEVICE;
+}
+
+int Batch::vjp(const DeviceModel &dm, const double *d_coords, const int *d_types, const double *d_g, double *d_gR,
+               float *d_gW, hipStream_t stream) {
+  BatchDev b = view(d_coords, d_types);
+  ModelDev md = dm.view();
+  NbDev nb = nb_view();
+  launch_binning(b, cell_tmp_.p, d_gR, stream);
+  launch_search(b, nb, stream);
+  OutDev o;
+  std::memset(&o, 0, sizeof(o));
+  o.E_atom = scratch_E_.p;
+  o.F = d_gR;
+  o.g_atom = d_g;
+  EnvDev e;
+  std::memset(&e, 0, sizeof(e));
+  const AtomSizes z = dm.sizes(b.natoms, nb_cap_);
+  if (dm.desc == DESC_CHEBYSHEV)
+    launch_atom_cheb(dm.max_rN, dm.max_aN, 0, true, z, md, b, nb, o, e, stream);
+  else
+    launch_atom_bp(0, true, z, md, b, nb, o, e, stream);
+  // the kernels accumulate forces, -gradient
+  launch_scale_f64(d_gR, -1.0, 3 * (long)b.natoms, stream);
+  if (d_gW) {
+    const size_t need = (size_t)b.natoms * dm.nsf_max;
+    if (scratch_G_.n < need && scratch_G_.alloc(need)) return AENET_STATUS_ERR_MALLOC;
+    OutDev od;
+    std::memset(&od, 0, sizeof(od));
+    od.E_atom = scratch_E_.p;
+    od.G = scratch_G_.p;
+    od.G_scaled = 1;
+    od.G_stride = dm.nsf_max;
+    od.desc_only = 1;
+    if (dm.desc == DESC_CHEBYSHEV)
+      launch_atom_cheb(dm.max_rN, dm.max_aN, 0, false, z, md, b, nb, od, e, stream);
+    else
+      launch_atom_bp(0, false, z, md, b, nb, od, e, stream);
+    NetGradArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.G = scratch_G_.p;
+    a.stride = dm.nsf_max;
+    a.types = d_types;
+    a.upstream = d_g;
+    a.grad = d_gW;
+    a.flags = flags_.p;
+    launch_net_grad(md, a, b.natoms, dm.nsf_max, dm.nl_max, dm.wsize, true, stream);
+  }
+  return hipGetLastError() == hipSuccess ? AENET_STATUS_OK : AENET_STATUS_ERR_DEVICE;
+}
+
+int Batch::eval_active(const DeviceModel &dm, const double *d_coords, const int *d_types, const int *d_active,
+                       const int *d_nactive, int max_active, bool reuse_lists, double *d_E_atom,
+                       hipStream_t stream) {
+  if (max_active <= 0) return AENET_STATUS_OK;
+  BatchDev b = view(d_coords, d_types);
+  ModelDev md = dm.view();
+  NbDev nb = nb_view();
+  if (!reuse_lists) {
+    launch_binning(b, cell_tmp_.p, nullptr, stream);
+    launch_search(b, nb, stream, d_active, d_nactive, max_active);
+  }
+  OutDev o;
+  std::memset(&o, 0, sizeof(o));
+  o.E_atom = d_E_atom;
+  EnvDev e;
+  std::memset(&e, 0, sizeof(e));
+  const AtomSizes z = dm.sizes(max_active, nb_cap_, d_active, d_nactive);
+  if (dm.desc == DESC_CHEBYSHEV)
+    launch_atom_cheb(dm.max_rN, dm.max_aN, 0, false, z, md, b, nb, o, e, stream);
+  else
+    launch_atom_bp(0, false, z, md, b, nb, o, e, stream);
   return hipGetLastError() == hipSuccess ? AENET_STATUS_OK : AENET_STATUS_ERR_DEVICE;
 }
 

@@ -50,7 +50,10 @@ struct DeviceModel {
   DBuf<BPDev> bp;
   DBuf<BPFunc> bpf;
   int ntypes = 0, desc = 0, max_rN = 0, max_aN = 0, nsf_max = 0, wsize = 0, nl_max = 0;
-  AtomSizes sizes(int natoms, int cap) const { return AtomSizes{natoms, cap, nsf_max, wsize, nl_max}; }
+  double rc_max = 0.0;
+  AtomSizes sizes(int natoms, int cap, const int *active = nullptr, const int *nactive = nullptr) const {
+    return AtomSizes{natoms, cap, nsf_max, wsize, nl_max, active, nactive};
+  }
   int upload(const PotSet &ps, hipStream_t s);
   ModelDev view() const;
 };
@@ -68,6 +71,19 @@ class Batch {
   // (total, cohesive); F may be null when forces are not wanted
   int eval(const DeviceModel &dm, const double *d_coords, const int *d_types, double *d_E_atom,
            double *d_E_struct, double *d_F, hipStream_t stream);
+  // atomic energies of the atoms d_active[0 .. *d_nactive-1] only (no
+  // forces, no structure sums); max_active bounds *d_nactive.  With
+  // reuse_lists the neighbor lists of the previous evaluation of these
+  // coordinates are used as they are (the atoms must have been searched).
+  int eval_active(const DeviceModel &dm, const double *d_coords, const int *d_types, const int *d_active,
+                  const int *d_nactive, int max_active, bool reuse_lists, double *d_E_atom, hipStream_t stream);
+  // vector-Jacobian product of the atomic energies: d_gR (3 per atom) =
+  // sum_i g_i dE_i/dR (overwritten); d_gW (wsize floats, may be null) +=
+  // sum_i g_i dE_i/dW
+  int vjp(const DeviceModel &dm, const double *d_coords, const int *d_types, const double *d_g, double *d_gR,
+          float *d_gW, hipStream_t stream);
+  // the model's cutoff must not exceed the one the batch was planned for
+  bool fits(const DeviceModel &dm) const { return dm.rc_max <= rc_max_ * (1.0 + 1e-12); }
   // bit-exact neighbor lists of every atom
   int neighbor_lists(const double *d_coords, const int *d_types, int cap, int *d_count, double *d_coo,
                      double *d_dist, int *d_list, int *d_type, hipStream_t stream);
@@ -91,6 +107,7 @@ class Batch {
   BatchDev view(const double *d_coords, const int *d_types) const;
   int allocate(std::string &err);
   PlanHost plan_;
+  double rc_max_ = 0.0;
   DBuf<StructDev> st_;
   DBuf<int> atom_struct_, cell_struct_, cell_ijk_, Tvec_, Cvec_, self_t_;
   DBuf<double> self_d_;
@@ -106,6 +123,7 @@ class Batch {
   bool timing_ = false;
   std::vector<hipEvent_t> ev_;
   DBuf<double> scratch_E_;
+  DBuf<double> scratch_G_;
 };
 
 // explicit environments (per-atom entry points)

@@ -10,13 +10,17 @@ namespace aenet_amd {
 // kernels.hip
 void launch_fill_i32(int *p, int v, long n, hipStream_t s);
 void launch_fill_f64(double *p, double v, long n, hipStream_t s);
+void launch_scale_f64(double *p, double a, long n, hipStream_t s);
 // (cell, image) combinations of every home cell; once per batch
 void launch_combos(const BatchDev &b, int *combo_count, int *combos, hipStream_t s);
 // cell binning of a batch (cell_count/cell_fill are cleared here); tmp: natoms ints
 // F (3 per atom, may be null) is cleared on the way
 void launch_binning(const BatchDev &b, int *tmp, double *F, hipStream_t s);
 // compact neighbor lists for the atom kernels
-void launch_search(const BatchDev &b, const NbDev &nb, hipStream_t s);
+// active (optional, device): search only atoms active[0 .. *nactive-1];
+// max_active bounds *nactive
+void launch_search(const BatchDev &b, const NbDev &nb, hipStream_t s, const int *active = nullptr,
+                   const int *nactive = nullptr, int max_active = 0);
 // bit-exact FP64 neighbor lists
 void launch_neighbor_lists(const BatchDev &b, const OutDev &o, hipStream_t s);
 void launch_struct_energy(const BatchDev &b, const ModelDev &md, const double *E_atom, double *E_tot,
@@ -37,6 +41,12 @@ void raise_flag(int *flags, int bits, hipStream_t s);
 void launch_atom_cheb(int rN, int aN, int mode, bool forces, const AtomSizes &z, const ModelDev &md,
                       const BatchDev &b, const NbDev &nb, const OutDev &o, const EnvDev &e,
                       hipStream_t stream);
+
+// train.hip
+void launch_net_grad(const ModelDev &md, const NetGradArgs &a, int natoms, int nsf_max, int nl_max, int wsize,
+                     bool grad, hipStream_t stream);
+void launch_adam(float *W, const float *g, float *m, float *v, int n, int t, float lr, float b1, float b2,
+                 float eps, hipStream_t stream);
 
 // atom_bp.hip
 void launch_atom_bp(int mode, bool forces, const AtomSizes &z, const ModelDev &md, const BatchDev &b,

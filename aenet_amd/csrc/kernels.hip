@@ -26,6 +26,11 @@ __global__ void k_fill_f64(double *p, double v, long n) {
   if (i < n) p[i] = v;
 }
 
+__global__ void k_scale_f64(double *p, double a, long n) {
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = a * p[i];
+}
+
 // ---------------------------------------------------------------------
 // binning: Cartesian -> lattice coordinates (geo_init_xsf,
 // src/geometry.f90:716), wrap (wrap_cooLatt, src/ext/lclist.f90:953-987),
@@ -285,10 +290,14 @@ struct CompactSink {
   }
 };
 
-__global__ __launch_bounds__(64) void k_search(BatchDev b, NbDev nb) {
+__global__ __launch_bounds__(64) void k_search(BatchDev b, NbDev nb, const int *active, const int *nactive) {
   __shared__ SearchLds W;
   __shared__ int ovf;
-  const int g = blockIdx.x;
+  int g = blockIdx.x;
+  if (active) {
+    if (g >= *nactive) return;  // the whole workgroup leaves together
+    g = active[g];
+  }
   if (threadIdx.x == 0) ovf = 0;
   __syncthreads();
   CompactSink sink{nb, g, &ovf};
@@ -382,6 +391,10 @@ void launch_fill_f64(double *p, double v, long n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_fill_f64, dim3(nblk(n, 256)), dim3(256), 0, s, p, v, n);
 }
 
+void launch_scale_f64(double *p, double a, long n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_scale_f64, dim3(nblk(n, 256)), dim3(256), 0, s, p, a, n);
+}
+
 void launch_combos(const BatchDev &b, int *combo_count, int *combos, hipStream_t s) {
   if (b.ncells > 0) hipLaunchKernelGGL(k_combos, dim3(b.ncells), dim3(256), 0, s, b, combo_count, combos);
 }
@@ -394,8 +407,10 @@ void launch_binning(const BatchDev &b, int *tmp, double *F, hipStream_t s) {
   hipLaunchKernelGGL(k_cell_sort, dim3(b.ncells), dim3(64), 0, s, b, (const int *)tmp);
 }
 
-void launch_search(const BatchDev &b, const NbDev &nb, hipStream_t s) {
-  if (b.natoms > 0) hipLaunchKernelGGL(k_search, dim3(b.natoms), dim3(64), 0, s, b, nb);
+void launch_search(const BatchDev &b, const NbDev &nb, hipStream_t s, const int *active, const int *nactive,
+                   int max_active) {
+  const int n = active ? max_active : b.natoms;
+  if (n > 0) hipLaunchKernelGGL(k_search, dim3(n), dim3(64), 0, s, b, nb, active, nactive);
 }
 
 void launch_neighbor_lists(const BatchDev &b, const OutDev &o, hipStream_t s) {

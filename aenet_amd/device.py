@@ -18,13 +18,17 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
 
 
+def _handle(model):
+    return model.handle if model is not None else None
+
+
 class DeviceBatch:
     """A fixed set of structures evaluated on the GPU.
 
     structures: list of objects with .cell (3x3, rows = vectors), .pbc,
     .positions (N,3) and .types (1-based ids of the loaded potentials)."""
 
-    def __init__(self, structures, device="cuda"):
+    def __init__(self, structures, device="cuda", model=None):
         self.device = torch.device(device)
         self.nstruct = len(structures)
         self.natoms_per = np.array([len(s.types) for s in structures], dtype=np.int32)
@@ -34,8 +38,9 @@ class DeviceBatch:
         types = np.ascontiguousarray(np.concatenate([np.asarray(s.types, np.int32) for s in structures]))
         h = C.c_void_p()
         stat = C.c_int(0)
-        _native.lib().aenet_amd_batch_create(self.nstruct, self.natoms_per, latt, pbc, coords.reshape(-1),
-                                             C.byref(h), C.byref(stat))
+        self.model = model
+        _native.lib().aenet_amd_batch_create_model(_handle(model), self.nstruct, self.natoms_per, latt, pbc,
+                                                   coords.reshape(-1), C.byref(h), C.byref(stat))
         check_status(stat.value, "aenet_amd_batch_create")
         self._h = h
         self.natoms = int(self.natoms_per.sum())
@@ -55,22 +60,56 @@ class DeviceBatch:
     def _stream(self):
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
-    def eval(self, coords=None, types=None, forces=True, check=True):
+    def _model(self, model):
+        return _handle(model if model is not None else self.model)
+
+    def _check_inputs(self, coords, types):
+        assert coords.is_contiguous() and coords.dtype == torch.float64 and coords.numel() == 3 * self.natoms
+        assert types.is_contiguous() and types.dtype == torch.int32 and types.numel() == self.natoms
+
+    def eval(self, coords=None, types=None, forces=True, check=True, model=None):
         """Launch one evaluation on the current torch stream; returns
         (E_tot, E_coh, E_atoms, forces) device tensors."""
         coords = self.coords if coords is None else coords
         types = self.types if types is None else types
-        assert coords.is_contiguous() and coords.dtype == torch.float64 and coords.numel() == 3 * self.natoms
-        assert types.is_contiguous() and types.dtype == torch.int32 and types.numel() == self.natoms
+        self._check_inputs(coords, types)
         stat = C.c_int(0)
-        _native.lib().aenet_amd_batch_eval(self._h, _ptr(coords), _ptr(types), _ptr(self.E_atoms),
-                                           _ptr(self.E_struct), _ptr(self.forces) if forces else None,
-                                           self._stream(), C.byref(stat))
+        _native.lib().aenet_amd_batch_eval_model(self._h, self._model(model), _ptr(coords), _ptr(types),
+                                                 _ptr(self.E_atoms), _ptr(self.E_struct),
+                                                 _ptr(self.forces) if forces else None, self._stream(),
+                                                 C.byref(stat))
         check_status(stat.value, "aenet_amd_batch_eval")
         if check:
             self.check()
         return (self.E_struct[:self.nstruct], self.E_struct[self.nstruct:], self.E_atoms,
                 self.forces if forces else None)
+
+    def eval_active(self, active, nactive, max_active, E_out, coords=None, types=None, reuse_lists=False,
+                    model=None):
+        """Atomic energies of the atoms active[:nactive[0]] into E_out
+        (device tensors: int32 active list, int32 count, float64 energies)."""
+        coords = self.coords if coords is None else coords
+        types = self.types if types is None else types
+        self._check_inputs(coords, types)
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_eval_active(self._h, self._model(model), _ptr(coords), _ptr(types),
+                                                  _ptr(active), _ptr(nactive), int(max_active), int(reuse_lists),
+                                                  _ptr(E_out), self._stream(), C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_eval_active")
+
+    def vjp(self, g, grad_W=None, coords=None, types=None, model=None):
+        """sum_i g_i dE_i/dR (returned, (natoms, 3) float64) and, when grad_W
+        is given, grad_W += sum_i g_i dE_i/dW."""
+        coords = self.coords if coords is None else coords
+        types = self.types if types is None else types
+        self._check_inputs(coords, types)
+        g = g.to(torch.float64).contiguous()
+        gR = torch.empty((self.natoms, 3), dtype=torch.float64, device=self.device)
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_vjp(self._h, self._model(model), _ptr(coords), _ptr(types), _ptr(g), _ptr(gR),
+                                          _ptr(grad_W), self._stream(), C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_vjp")
+        return gR
 
     def check(self):
         stat = C.c_int(0)
@@ -106,13 +145,13 @@ class DeviceBatch:
         self.check()
         return {"nnb": count, "coo": coo, "dist": dist, "list": lst, "type": typ}
 
-    def descriptors(self, stride, scaled=False, coords=None, types=None):
+    def descriptors(self, stride, scaled=False, coords=None, types=None, model=None):
         coords = self.coords if coords is None else coords
         types = self.types if types is None else types
         G = torch.zeros((self.natoms, stride), dtype=torch.float64, device=self.device)
         stat = C.c_int(0)
-        _native.lib().aenet_amd_batch_descriptors(self._h, _ptr(coords), _ptr(types), int(scaled), _ptr(G), stride,
-                                                  self._stream(), C.byref(stat))
+        _native.lib().aenet_amd_batch_descriptors_model(self._h, self._model(model), _ptr(coords), _ptr(types),
+                                                        int(scaled), _ptr(G), stride, self._stream(), C.byref(stat))
         check_status(stat.value, "aenet_amd_batch_descriptors")
         self.check()
         return G

@@ -48,8 +48,9 @@ void aenet_amd_batch_create(int nstruct, const int natoms[], const double latt[]
 void aenet_amd_batch_destroy(aenet_amd_batch *batch);
 int aenet_amd_batch_natoms(const aenet_amd_batch *batch);
 
-/* d_* are device pointers; stream is a hipStream_t (NULL: the library's
- * stream).  d_E_struct holds 2*nstruct doubles: E_tot then E_coh.  d_F
+/* d_* are device pointers; stream is the hipStream_t every entry point
+ * with device pointers launches on (NULL: the default stream, as in HIP),
+ * so work is ordered with the caller's own.  d_E_struct holds 2*nstruct doubles: E_tot then E_coh.  d_F
  * (3 per atom) is overwritten; NULL skips forces.  The call is
  * asynchronous; aenet_amd_batch_check synchronizes and reports capacity
  * or type errors raised by the kernels. */
@@ -81,6 +82,87 @@ void aenet_amd_batch_neighbors(aenet_amd_batch *batch, const double *d_coords,
 void aenet_amd_batch_descriptors(aenet_amd_batch *batch, const double *d_coords,
                                  const int *d_types, int scaled, double *d_G,
                                  int stride, void *stream, int *stat);
+
+/* ------------------------------------------------------------------
+ * Independent potential sets.  The reference keeps one set of potentials
+ * in module globals (src/aenet.f90:122-132); a caller that needs two at
+ * once -- a cheap model and the full one in delayed-acceptance Monte
+ * Carlo, or a network under training beside its teacher -- loads them as
+ * handles.  Passing NULL as the model of an entry point below means the
+ * set loaded with aenet_load_potential.
+ * ------------------------------------------------------------------ */
+typedef struct aenet_amd_model aenet_amd_model;
+
+/* files[t] is the .nn potential of species atom_types[t] (type id t+1). */
+void aenet_amd_model_load(int ntypes, char *atom_types[], char *files[],
+                          aenet_amd_model **model, int *stat);
+void aenet_amd_model_destroy(aenet_amd_model *model);
+void aenet_amd_model_info(const aenet_amd_model *model, double *Rc_max,
+                          int *nsf_max, int *wsize, int *stat);
+/* offset and length of species type_id's network in the weight array
+ * (the .nn layout: per layer a column-major (n_out, n_in+1) block, bias
+ * last) */
+void aenet_amd_model_species_weights(const aenet_amd_model *model, int type_id,
+                                     int *offset, int *size, int *stat);
+/* d_W: wsize floats on the device; to_model = 0 copies the model's
+ * weights out, 1 replaces them */
+void aenet_amd_model_weights(aenet_amd_model *model, float *d_W, int to_model,
+                             void *stream, int *stat);
+
+/* a batch planned for the model's cutoff */
+void aenet_amd_batch_create_model(const aenet_amd_model *model, int nstruct,
+                                  const int natoms[], const double latt[],
+                                  const int pbc[], const double coords_host[],
+                                  aenet_amd_batch **batch, int *stat);
+/* aenet_amd_batch_eval with a given model (its cutoff must not exceed the
+ * batch's) */
+void aenet_amd_batch_eval_model(aenet_amd_batch *batch, const aenet_amd_model *model,
+                                const double *d_coords, const int *d_types,
+                                double *d_E_atoms, double *d_E_struct, double *d_F,
+                                void *stream, int *stat);
+/* Atomic energies of a subset of atoms: d_active[0 .. *d_nactive-1]
+ * (device), max_active >= *d_nactive.  Only those entries of d_E_atoms
+ * are written.  With reuse_lists != 0 the neighbor lists built by the
+ * previous evaluation of the same coordinates are used (the subset must
+ * have been part of that evaluation).  For local energy changes in Monte
+ * Carlo. */
+void aenet_amd_batch_eval_active(aenet_amd_batch *batch, const aenet_amd_model *model,
+                                 const double *d_coords, const int *d_types,
+                                 const int *d_active, const int *d_nactive,
+                                 int max_active, int reuse_lists, double *d_E_atoms,
+                                 void *stream, int *stat);
+/* Vector-Jacobian product of the atomic energies E_i of a batch, for
+ * reverse-mode differentiation: d_grad_coords (3 per atom, overwritten)
+ * = sum_i g_i dE_i/dR and, if d_grad_W is not NULL, d_grad_W (wsize
+ * floats) += sum_i g_i dE_i/dW.  d_g: natoms doubles (device). */
+void aenet_amd_batch_vjp(aenet_amd_batch *batch, const aenet_amd_model *model,
+                         const double *d_coords, const int *d_types, const double *d_g,
+                         double *d_grad_coords, float *d_grad_W, void *stream, int *stat);
+void aenet_amd_batch_descriptors_model(aenet_amd_batch *batch, const aenet_amd_model *model,
+                                       const double *d_coords, const int *d_types,
+                                       int scaled, double *d_G, int stride,
+                                       void *stream, int *stat);
+
+/* Network training on normalized descriptors (d_G: natoms*stride, as
+ * aenet_amd_batch_descriptors_model(scaled=1) writes them).  Computes the
+ * atomic energies (d_E_out, optional) and adds to d_grad (wsize floats)
+ * the gradient of sum_i g_i E_i with respect to the weights -- the
+ * reference's ff_wderiv (src/ext/feedforward.f90:763-860) times E_scale,
+ * contracted with g.  g_i = coef * weight_i * (E_i - E_ref_i) when d_E_ref
+ * is given (then d_loss, optional, receives += sum weight_i (E_i -
+ * E_ref_i)^2), else g_i = d_upstream[i].  d_grad NULL: energies only. */
+void aenet_amd_model_net_grad(aenet_amd_model *model, int natoms, const double *d_G,
+                              int stride, const int *d_types, const double *d_E_ref,
+                              const double *d_weight, const double *d_upstream,
+                              double coef, double *d_E_out, float *d_grad,
+                              double *d_loss, void *stream, int *stat);
+/* One Adam step on the model's weights (src/optimize.f90:1012-1046);
+ * d_m, d_v: wsize floats, zero before the first step; t >= 1. */
+void aenet_amd_model_adam(aenet_amd_model *model, const float *d_grad, float *d_m,
+                          float *d_v, int t, double lr, double b1, double b2,
+                          double eps, void *stream, int *stat);
+/* synchronizes and reports errors raised by the training kernels */
+void aenet_amd_model_check(aenet_amd_model *model, void *stream, int *stat);
 
 /* Text of the last error of the calling thread. */
 const char *aenet_amd_last_error(void);

@@ -1,0 +1,182 @@
+"""Hand-written backward kernels: network weight gradients, the Adam step
+and the torch.autograd path through the energies.
+
+- weight gradients of single atoms against the oracle's ff_wderiv
+  (src/ext/feedforward.f90:763-860) on the same normalized descriptors;
+- the squared-error gradient against central differences of the loss,
+  and Adam against the oracle's restatement of src/optimize.f90:1012-1046;
+- autograd: -dE/dR equals the force kernels' forces; sum_i g_i dE_i/dR for
+  random per-atom g against the oracle's per-atom analytic forces; central
+  differences of the energy in the coordinates and in the weights.
+"""
+import numpy as np
+import pytest
+
+from conftest import pot_files
+
+pytestmark = pytest.mark.gpu
+
+
+def _model_and_batch(device, name="alloy", types=("A", "B"), structs=None):
+    from aenet_amd import synth
+    from aenet_amd.device import DeviceBatch
+    from aenet_amd.model import Model
+    m = Model(pot_files(name, list(types)), device=device)
+    if structs is None:
+        structs = [synth.binary_fcc(3.8, (3, 3, 3), 0.08, 31, 32)]
+    db = DeviceBatch(structs, device=device, model=m)
+    return m, db
+
+
+def test_weight_gradient_matches_oracle(device):
+    import torch
+    from aenet_amd import nnfile
+    from oracle_lib import Oracle
+    names = ["A", "B"]
+    m, db = _model_and_batch(device)
+    G = db.descriptors(m.nsf_max, scaled=True)
+    types = db.types
+    n = db.natoms
+    o = Oracle(names, pot_files("alloy", names))
+    picks = [0, 1, 7, n - 1]
+    for i in picks:
+        t = int(types[i].item())
+        off, size = m.species_weights(t)
+        escale = nnfile.read_nn(pot_files("alloy", names)[names[t - 1]])["E_scale"]
+        up = torch.zeros(n, dtype=torch.float64, device=device)
+        up[i] = 1.0
+        grad = torch.zeros(m.wsize, dtype=torch.float32, device=device)
+        E = torch.zeros(n, dtype=torch.float64, device=device)
+        m.net_grad(G, types, upstream=up, E_out=E, grad=grad)
+        torch.cuda.synchronize()
+        x = G[i, :o.nsf(t)].cpu().numpy()
+        y, _, dw = o.network(t, x, size)
+        g = grad.cpu().numpy()
+        ref = escale * dw
+        err = np.abs(g[off:off + size] - ref).max() / max(1.0, np.abs(ref).max())
+        print("atom %d type %d: max rel |d grad| %.3e" % (i, t, err))
+        assert err < 2e-5
+        assert np.all(g[:off] == 0) and np.all(g[off + size:] == 0)
+    m.check()
+
+
+def test_mse_gradient_fd_and_adam(device):
+    import torch
+    from oracle_lib import Oracle
+    m, db = _model_and_batch(device)
+    G = db.descriptors(m.nsf_max, scaled=True)
+    types = db.types
+    n = db.natoms
+    gen = torch.Generator(device=device).manual_seed(3)
+    E0 = torch.zeros(n, dtype=torch.float64, device=device)
+    m.net_grad(G, types, E_out=E0)
+    E_ref = E0 + 0.05 * torch.randn(n, generator=gen, device=device, dtype=torch.float64)
+    coef = 2.0 / n
+
+    def loss_at(W):
+        m.set_weights(W)
+        L = torch.zeros(1, dtype=torch.float64, device=device)
+        m.net_grad(G, types, E_ref=E_ref, coef=coef, loss=L)
+        return float(L.item()) / n
+
+    W0 = m.weights()
+    grad = torch.zeros(m.wsize, dtype=torch.float32, device=device)
+    L = torch.zeros(1, dtype=torch.float64, device=device)
+    m.net_grad(G, types, E_ref=E_ref, coef=coef, grad=grad, loss=L)
+    g = grad.cpu().numpy()
+    print("grad: %d nonzero, max |g| %.3e, loss %.4e" % (np.count_nonzero(g), np.abs(g).max(), L.item() / n))
+    assert np.all(np.isfinite(g)) and np.abs(g).max() > 0
+    order = np.argsort(-np.abs(g))[:6]
+    for k in order:
+        h = 1e-3
+        Wp = W0.clone()
+        Wp[k] += h
+        Wm = W0.clone()
+        Wm[k] -= h
+        fd = (loss_at(Wp) - loss_at(Wm)) / (2 * h)
+        print("w[%d]: kernel %.6e  fd %.6e" % (k, g[k], fd))
+        assert abs(fd - g[k]) <= 2e-3 * abs(g[k]) + 1e-6
+    m.set_weights(W0)
+    # Adam: two steps against the oracle's restatement
+    mom = torch.zeros(m.wsize, dtype=torch.float32, device=device)
+    vel = torch.zeros_like(mom)
+    W = W0.cpu().numpy().astype(np.float64)
+    mm = np.zeros_like(W)
+    vv = np.zeros_like(W)
+    lib = Oracle.lib()
+    for t in (1, 2):
+        m.adam(grad, mom, vel, t, lr=1e-3)
+        lib.orc_adam_step(len(W), W, g.astype(np.float64), mm, vv, t, 1e-3, 0.9, 0.999, 1e-8)
+    Wg = m.weights().cpu().numpy()
+    print("adam max |dW| %.3e" % np.abs(Wg - W).max())
+    assert np.abs(Wg - W).max() < 1e-6
+    m.set_weights(W0)
+
+
+def test_autograd_forces_and_weights(device):
+    import torch
+    from aenet_amd import autograd
+    from oracle_lib import Oracle
+    names = ["A", "B"]
+    m, db = _model_and_batch(device)
+    x = db.coords.view(-1, 3).clone()
+    _, _, _, F = db.eval(coords=x.view(-1))
+    F = F.clone()
+    Fa = autograd.forces(db, x)
+    print("autograd vs force kernel max |dF| %.3e" % (Fa - F).abs().max().item())
+    assert torch.allclose(Fa, F, rtol=0, atol=1e-10)
+    # per-atom weights g: sum_i g_i dE_i/dR against the oracle's analytic forces
+    n = db.natoms
+    gen = torch.Generator(device=device).manual_seed(9)
+    gi = torch.randn(n, generator=gen, device=device, dtype=torch.float64)
+    xr = x.clone().requires_grad_(True)
+    _, E_at = autograd.energies(db, xr)
+    (gR,) = torch.autograd.grad((E_at * gi).sum(), xr)
+    from aenet_amd import synth
+    o = Oracle(names, pot_files("alloy", names))
+    s = synth.binary_fcc(3.8, (3, 3, 3), 0.08, 31, 32)
+    nb = o.neighbors(s.cell, True, s.types, s.positions)
+    Lr = nb["latt"]  # rows: lattice vectors
+    ref = np.zeros((n, 3))
+    gnp = gi.cpu().numpy()
+    for i in range(n):
+        Fi = np.zeros((n, 3))
+        k = nb["nnb"][i]
+        o.atomic_energy_and_forces(nb["frac"][i] @ Lr, int(s.types[i]), i + 1, nb["coo"][i, :k], nb["type"][i, :k],
+                                   nb["list"][i, :k], Fi)
+        ref -= gnp[i] * Fi
+    err = np.abs(gR.cpu().numpy() - ref).max()
+    print("sum_i g_i dE_i/dR vs oracle max %.3e" % err)
+    assert err < 2e-4
+    # central differences of E_tot in the coordinates (FP32 energies: h = 5e-3 A)
+    h = 5e-3
+    for (a, c) in [(0, 0), (5, 1), (17, 2)]:
+        xp = x.clone()
+        xp[a, c] += h
+        xm = x.clone()
+        xm[a, c] -= h
+        Ep = db.eval(coords=xp.view(-1).contiguous(), forces=False)[0].item()
+        Em = db.eval(coords=xm.view(-1).contiguous(), forces=False)[0].item()
+        fd = -(Ep - Em) / (2 * h)
+        print("F[%d,%d] = %.6f  fd %.6f" % (a, c, F[a, c].item(), fd))
+        assert abs(fd - F[a, c].item()) < 2e-3
+    # weights: gradient of E_tot against central differences
+    W0 = m.weights()
+    Wr = W0.clone().requires_grad_(True)
+    E_tot, _ = autograd.energies(db, x, weights=Wr, model=m)
+    (gW,) = torch.autograd.grad(E_tot.sum(), Wr)
+    gWn = gW.cpu().numpy()
+    print("grad W: %d nonzero, max %.3e" % (np.count_nonzero(gWn), np.abs(gWn).max()))
+    assert np.abs(gWn).max() > 0
+    for k in np.argsort(-np.abs(gWn))[:4]:
+        hw = 1e-3
+        Wp = W0.clone()
+        Wp[k] += hw
+        Wm = W0.clone()
+        Wm[k] -= hw
+        Ep = autograd.energies(db, x, weights=Wp, model=m)[0].sum().item()
+        Em = autograd.energies(db, x, weights=Wm, model=m)[0].sum().item()
+        fd = (Ep - Em) / (2 * hw)
+        print("dE/dW[%d] = %.6f  fd %.6f" % (k, gWn[k], fd))
+        assert abs(fd - gWn[k]) <= 5e-3 * abs(gWn[k]) + 1e-3
+    m.set_weights(W0)
