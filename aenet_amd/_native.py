@@ -73,6 +73,19 @@ SIGNATURES = {
     "aenet_amd_model_adam": (None, [_vp, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, C.c_double,
                                     C.c_double, _vp, _pint]),
     "aenet_amd_model_check": (None, [_vp, _vp, _pint]),
+    "aenet_amd_mc_create": (None, [_vp, _vp, C.c_int, C.c_int, _dp, _ip, _dp, C.c_ulonglong, C.POINTER(_vp), _pint]),
+    "aenet_amd_mc_destroy": (None, [_vp]),
+    "aenet_amd_mc_set": (None, [_vp, C.c_double, C.c_double, C.c_double, _pint]),
+    "aenet_amd_mc_refresh": (None, [_vp, _pint]),
+    "aenet_amd_mc_run": (None, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _pint]),
+    "aenet_amd_mc_stats": (None, [_vp, C.POINTER(C.c_longlong), C.c_int, _pint]),
+    "aenet_amd_mc_energies": (None, [_vp, _dp, _pint]),
+    "aenet_amd_mc_samples": (None, [_vp, _vp, C.POINTER(C.c_longlong), _pint]),
+    "aenet_amd_mc_state": (None, [_vp, _vp, _vp, _pint]),
+    "aenet_amd_mc_trace": (None, [_vp, C.c_longlong, _vp, C.POINTER(C.c_longlong), _pint]),
+    "aenet_amd_mc_first_stage_state": (None, [_vp, _vp, _vp, _pint, _pint]),
+    "aenet_amd_model_fit_cells": (None, [_vp, C.c_int, C.c_int, _dp, _vp, _vp, _vp, C.c_int, _vp, _vp, C.c_double,
+                                         C.c_double, _vp, _vp, _vp, _pint]),
     "aenet_amd_last_error": (C.c_char_p, []),
 }
 

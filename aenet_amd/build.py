@@ -20,8 +20,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("AENET_AMD_ARCH", "gfx950")
 
-DEVICE_SOURCES = ["kernels.hip", "atom_cheb.hip", "atom_bp.hip", "train.hip"]
-HOST_SOURCES = ["model.cpp", "plan.cpp", "engine.cpp", "capi.cpp"]
+DEVICE_SOURCES = ["kernels.hip", "atom_cheb.hip", "atom_bp.hip", "train.hip", "mc.hip"]
+HOST_SOURCES = ["model.cpp", "plan.cpp", "engine.cpp", "mc.cpp", "capi.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + CSRC, "-I" + os.path.join(os.path.dirname(HERE), "include")]
 DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -46,11 +46,11 @@ def build(verbose=True, jobs=8):
     cmds = []
     objs = []
     for src in DEVICE_SOURCES:
-        o = os.path.join(obj_dir, src.replace(".hip", ".o"))
+        o = os.path.join(obj_dir, src + ".o")
         cmds.append([HIPCC, "-x", "hip", *COMMON, *DEVICE_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     for src in HOST_SOURCES:
-        o = os.path.join(obj_dir, src.replace(".cpp", ".o"))
+        o = os.path.join(obj_dir, src + ".o")
         cmds.append([CXX, *COMMON, *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     procs = []

@@ -188,3 +188,38 @@ struct EnvDev {
   double *dF_i;           // [3*nenv] force on the central atom
   double *dF_j;           // [3*total] force on each neighbor entry
 };
+
+// Monte Carlo of independent periodic chains (mc.hip): nchains cells of
+// natoms atoms each, all with the same lattice
+enum { MC_NONE = 0, MC_SWAP = 1, MC_DISP = 2 };
+
+struct MCDev {
+  int nchains, natoms;
+  double L[9];      // rows: lattice vectors (the same cell for every chain)
+  double Binv[9];   // inverse of L (fractional = v * Binv)
+  double *coords;   // [3*nchains*natoms] Cartesian
+  int *types;       // [nchains*natoms]
+  unsigned long long *rng;  // [nchains] generator states
+  double beta, p_swap, max_disp, rc_t;
+  // the step's move per chain
+  int *kind, *site_a, *site_b;
+  double *old_pos;  // [3*nchains]
+  double *u_stage1, *u_stage2;
+  int *need;        // chain goes to the full potential
+  double *dE_s;     // first-stage energy change
+  // atoms re-evaluated by the full potential, one block per chain
+  int *nactive, *active, *act_start, *act_count;
+  // full potential: atomic energies (current, trial), chain energies
+  double *E_t, *E_trial, *E_chain;
+  // first-stage potential: unscaled descriptors and atomic energies
+  int nsf_s;
+  double *G_s, *G_trial, *E_s, *E_s_trial;
+  int *affected;
+  long long *stats;  // [4*nchains]: proposed, accepted, full evaluations, swaps
+  long long *step;
+  double *trace;     // optional [max_trace*nchains*4]: dE_first, dE_full, passed stage 1, accepted
+  long long max_trace;
+  double *samples;   // [max_samples*nchains] chain energies every sample_every steps
+  int sample_every;
+  long long max_samples;
+};

@@ -16,6 +16,7 @@
 
 #include "aenet_capi_status.h"
 #include "kernels.h"
+#include "mc.h"
 #include "engine.h"
 #include "model.h"
 
@@ -137,6 +138,10 @@ struct aenet_amd_model {
   PotSet set;
   DeviceModel dev;
   DBuf<int> flags;  // error word of the training kernels
+};
+
+struct aenet_amd_mc {
+  MonteCarlo mc;
 };
 
 namespace {
@@ -1022,6 +1027,166 @@ void aenet_amd_model_check(aenet_amd_model *model, void *stream, int *stat) {
     return;
   }
   *stat = AENET_STATUS_OK;
+}
+
+
+// ---------------------------------------------------------------------
+// Monte Carlo (include/aenet_amd.h)
+// ---------------------------------------------------------------------
+
+void aenet_amd_mc_create(const aenet_amd_model *full, const aenet_amd_model *first, int nchains, int natoms,
+                         const double latt[9], const int types[], const double coords[], unsigned long long seed,
+                         aenet_amd_mc **mc, int *stat) {
+  *mc = nullptr;
+  if (!full) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "aenet_amd_mc_create: no full potential");
+    return;
+  }
+  if (first) {
+    const PotSet &s = first->set;
+    bool ok = s.desc == DESC_CHEBYSHEV && s.ntypes == full->set.ntypes && s.nsf_max <= 32;
+    for (const SpeciesDev &sp : s.species) {
+      ok = ok && sp.a_Rc == 0.0f && sp.nlayers >= 2 && sp.nnodes[sp.nlayers - 1] == 1;
+      for (int l = 0; l < sp.nlayers; ++l) ok = ok && sp.nnodes[l] <= 32;
+    }
+    if (!ok) {
+      *stat = set_err(AENET_AMD_ERR_ARG_VALUE,
+                      "first-stage potential must be radial-only Chebyshev (angular Rc 0) with at most 32 "
+                      "features and layers of at most 32 nodes, for the same species as the full potential");
+      return;
+    }
+  }
+  std::unique_ptr<aenet_amd_mc> m(new aenet_amd_mc());
+  std::string err;
+  int st = m->mc.create(full->set, &full->dev, first ? &first->dev : nullptr, nchains, natoms, latt, types, coords,
+                        seed, err);
+  if (st) {
+    *stat = set_err(st, err);
+    return;
+  }
+  *mc = m.release();
+  *stat = AENET_STATUS_OK;
+}
+
+void aenet_amd_mc_destroy(aenet_amd_mc *mc) { delete mc; }
+
+void aenet_amd_mc_set(aenet_amd_mc *mc, double beta, double p_swap, double max_disp, int *stat) {
+  if (!mc || !(beta >= 0.0) || !(p_swap >= 0.0 && p_swap <= 1.0) || !(max_disp >= 0.0)) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_amd_mc_set: bad arguments");
+    return;
+  }
+  mc->mc.set_params(beta, p_swap, max_disp);
+  *stat = AENET_STATUS_OK;
+}
+
+void aenet_amd_mc_refresh(aenet_amd_mc *mc, int *stat) {
+  if (!mc) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no Monte Carlo state");
+    return;
+  }
+  std::string err;
+  *stat = mc->mc.refresh(err);
+  if (*stat) set_err(*stat, err);
+}
+
+void aenet_amd_mc_run(aenet_amd_mc *mc, int nsteps, int delayed, int sample_every, int use_graph, int *stat) {
+  if (!mc) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no Monte Carlo state");
+    return;
+  }
+  std::string err;
+  *stat = mc->mc.run(nsteps, delayed != 0, sample_every, use_graph != 0, err);
+  if (*stat) set_err(*stat, err);
+}
+
+void aenet_amd_mc_stats(aenet_amd_mc *mc, long long out[4], int reset, int *stat) {
+  if (!mc) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no Monte Carlo state");
+    return;
+  }
+  *stat = mc->mc.stats(out);
+  if (!*stat && reset) mc->mc.reset_stats();
+  if (*stat) set_err(*stat, "device error");
+}
+
+void aenet_amd_mc_energies(aenet_amd_mc *mc, double E_chain[], int *stat) {
+  *stat = mc ? mc->mc.energies(E_chain) : set_err(AENET_STATUS_ERR_INIT, "no Monte Carlo state");
+}
+
+void aenet_amd_mc_samples(aenet_amd_mc *mc, double out[], long long *nsamples, int *stat) {
+  if (!mc) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no Monte Carlo state");
+    return;
+  }
+  long long n = mc->mc.samples(out);
+  *nsamples = n < 0 ? 0 : n;
+  *stat = n < 0 ? set_err(AENET_STATUS_ERR_DEVICE, "device error") : AENET_STATUS_OK;
+}
+
+void aenet_amd_mc_trace(aenet_amd_mc *mc, long long max_steps, double out[], long long *nsteps, int *stat) {
+  if (!mc) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no Monte Carlo state");
+    return;
+  }
+  if (max_steps >= 0) {
+    *stat = mc->mc.set_trace(max_steps);
+    if (*stat) set_err(*stat, "device allocation failed");
+    if (nsteps) *nsteps = 0;
+    return;
+  }
+  long long n = mc->mc.trace(out);
+  if (nsteps) *nsteps = n < 0 ? 0 : n;
+  *stat = n < 0 ? set_err(AENET_STATUS_ERR_DEVICE, "device error") : AENET_STATUS_OK;
+}
+
+void aenet_amd_mc_state(aenet_amd_mc *mc, double coords[], int types[], int *stat) {
+  *stat = mc ? mc->mc.state(coords, types) : set_err(AENET_STATUS_ERR_INIT, "no Monte Carlo state");
+}
+
+void aenet_amd_mc_first_stage_state(aenet_amd_mc *mc, double E_atoms[], double G[], int *nsf, int *stat) {
+  if (!mc) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no Monte Carlo state");
+    return;
+  }
+  if (nsf) *nsf = mc->mc.nsf_student();
+  *stat = mc->mc.student_state(E_atoms, G);
+}
+
+void aenet_amd_model_fit_cells(aenet_amd_model *model, int ncells, int natoms, const double latt[9],
+                               const double *d_coords, const int *d_types, const int *d_sel, int nsel,
+                               const double *d_E_ref, const double *d_F_ref, double coefE, double coefF,
+                               float *d_grad, double *d_loss, void *stream, int *stat) {
+  if (!model || !d_coords || !d_types || !d_sel || !d_E_ref || !d_F_ref || !d_grad || !d_loss || ncells < 1 ||
+      natoms < 2 || natoms > 256 || nsel < 0) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_amd_model_fit_cells: bad arguments");
+    return;
+  }
+  const PotSet &s = model->set;
+  bool ok = s.desc == DESC_CHEBYSHEV;
+  for (const SpeciesDev &sp : s.species)
+    ok = ok && sp.a_Rc == 0.0f && sp.r_N <= 16 && sp.nlayers == 3 && sp.nnodes[1] <= 16 && sp.nnodes[2] == 1 &&
+         sp.fa[2] == 0;
+  if (!ok) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE,
+                    "fit: radial-only Chebyshev basis (at most 16 terms), one hidden layer of at most 16 nodes, "
+                    "linear output");
+    return;
+  }
+  MCDev m;
+  std::memset(&m, 0, sizeof(m));
+  m.nchains = ncells;
+  m.natoms = natoms;
+  std::memcpy(m.L, latt, sizeof(m.L));
+  if (!mc_invert_cell(m.L, m.Binv) || mc_min_width(m.L) <= 2.0 * s.Rc_max) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "fit: singular cell or cell narrower than twice the cutoff");
+    return;
+  }
+  m.coords = const_cast<double *>(d_coords);
+  m.types = const_cast<int *>(d_types);
+  hipStream_t st = (hipStream_t)stream;  // NULL: the default stream
+  launch_first_stage_fit(m, model->dev.view(), d_sel, nsel, d_E_ref, d_F_ref, coefE, coefF, d_grad, model->dev.wsize,
+                         d_loss, st);
+  *stat = hipGetLastError() == hipSuccess ? AENET_STATUS_OK : set_err(AENET_STATUS_ERR_DEVICE, "launch failed");
 }
 
 }  // extern "C"

@@ -99,6 +99,10 @@ class Batch {
   ~Batch();
   int natoms() const { return plan_.natoms; }
   int nb_cap() const { return nb_cap_; }
+  // the batch's compact neighbor lists and error word, for callers that
+  // fill the lists themselves (eval_active with reuse_lists)
+  NbDev nb_lists() const { return nb_view(); }
+  int *flags_ptr() const { return flags_.p; }
   int set_nb_cap(int cap);
   int nstruct() const { return (int)plan_.st.size(); }
   const PlanHost &plan() const { return plan_; }

@@ -48,6 +48,20 @@ void launch_net_grad(const ModelDev &md, const NetGradArgs &a, int natoms, int n
 void launch_adam(float *W, const float *g, float *m, float *v, int n, int t, float lr, float b1, float b2,
                  float eps, hipStream_t stream);
 
+// mc.hip: phase 0 = propose (+ first stage) + active lists, phase 1 = finish
+void launch_mc_step(const MCDev &m, const ModelDev *student, bool plain, int phase, hipStream_t s);
+void launch_mc_student_init(const MCDev &m, const ModelDev &student, hipStream_t s);
+// neighbor lists (list radius rc_list) of the active atoms by minimum image
+void launch_mc_neighbors(const MCDev &m, const NbDev &nb, double rc_list, int *flags, hipStream_t s);
+void launch_mc_all_active(const MCDev &m, hipStream_t s);
+void launch_mc_chain_sum(const MCDev &m, hipStream_t s);
+// fit of the first-stage potential on cells (m: coords/types of the cells,
+// cells[ncells] their indices); grad += gradient, loss[0..1] += squared
+// energy and force residuals
+void launch_first_stage_fit(const MCDev &m, const ModelDev &md, const int *cells, int ncells, const double *E_ref,
+                            const double *F_ref, double coefE, double coefF, float *grad, int wsize, double *loss,
+                            hipStream_t s);
+
 // atom_bp.hip
 void launch_atom_bp(int mode, bool forces, const AtomSizes &z, const ModelDev &md, const BatchDev &b,
                     const NbDev &nb, const OutDev &o, const EnvDev &e, hipStream_t stream);

@@ -11,6 +11,7 @@ species).
 """
 import ctypes as C
 
+import numpy as np
 import torch
 
 from . import _native
@@ -84,6 +85,19 @@ class Model:
                                                _ptr(E_ref), _ptr(weight), _ptr(upstream), float(coef), _ptr(E_out),
                                                _ptr(grad), _ptr(loss), self._stream(), C.byref(stat))
         check_status(stat.value, "aenet_amd_model_net_grad")
+
+    def fit_cells(self, latt, natoms, coords, types, sel, E_ref, F_ref, coefE, coefF, grad, loss):
+        """grad += gradient of coefE/2 sum (E - E_ref)^2 + coefF/2 sum |F -
+        F_ref|^2 over the cells sel (device int32) of a set of cells of
+        natoms atoms with one lattice (rows = vectors); loss[0:2] += the two
+        sums of squares (aenet_amd_model_fit_cells)."""
+        stat = C.c_int()
+        L = np.ascontiguousarray(latt, dtype=np.float64).reshape(9)
+        _native.lib().aenet_amd_model_fit_cells(self.handle, int(types.numel() // natoms), int(natoms), L,
+                                                _ptr(coords), _ptr(types), _ptr(sel), int(sel.numel()), _ptr(E_ref),
+                                                _ptr(F_ref), float(coefE), float(coefF), _ptr(grad), _ptr(loss),
+                                                self._stream(), C.byref(stat))
+        check_status(stat.value, "aenet_amd_model_fit_cells")
 
     def adam(self, grad, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8):
         stat = C.c_int()

@@ -164,6 +164,70 @@ void aenet_amd_model_adam(aenet_amd_model *model, const float *d_grad, float *d_
 /* synchronizes and reports errors raised by the training kernels */
 void aenet_amd_model_check(aenet_amd_model *model, void *stream, int *stat);
 
+/* ------------------------------------------------------------------
+ * Metropolis Monte Carlo of independent periodic chains on the GPU: nchains
+ * cells of natoms atoms (latt: 9 doubles, rows = lattice vectors, the
+ * same cell for every chain; types and coords on the host, chains
+ * concatenated).  Moves: swap of two atoms of different species
+ * (probability p_swap) or displacement of one atom by U(-max_disp,
+ * max_disp)^3; acceptance at inverse temperature beta (1/eV).  With a
+ * first-stage potential (radial-only Chebyshev, small network) the run
+ * can use delayed acceptance: proposals are screened with the first
+ * stage's energy change (evaluated incrementally, no neighbor search), and
+ * only those that pass are decided by the full potential with
+ * min(1, exp(-beta (dE_full - dE_first))), which keeps the chain exact for
+ * the full potential.  The full potential re-evaluates only the atoms
+ * within its cutoff of the moved sites.  Each step's kernels can be
+ * captured in a HIP graph and replayed.  No counterpart in the
+ * reference. */
+typedef struct aenet_amd_mc aenet_amd_mc;
+
+void aenet_amd_mc_create(const aenet_amd_model *full, const aenet_amd_model *first,
+                         int nchains, int natoms, const double latt[9], const int types[],
+                         const double coords[], unsigned long long seed,
+                         aenet_amd_mc **mc, int *stat);
+void aenet_amd_mc_destroy(aenet_amd_mc *mc);
+void aenet_amd_mc_set(aenet_amd_mc *mc, double beta, double p_swap, double max_disp,
+                      int *stat);
+/* energies (and first-stage descriptors) recomputed from scratch */
+void aenet_amd_mc_refresh(aenet_amd_mc *mc, int *stat);
+/* nsteps moves of every chain (synchronous); delayed != 0: two-stage;
+ * sample_every > 0 records every chain's energy each sample_every steps */
+void aenet_amd_mc_run(aenet_amd_mc *mc, int nsteps, int delayed, int sample_every,
+                      int use_graph, int *stat);
+/* sums over chains: proposed, accepted, full-potential evaluations, swaps
+ * proposed; reset != 0 clears them */
+void aenet_amd_mc_stats(aenet_amd_mc *mc, long long out[4], int reset, int *stat);
+void aenet_amd_mc_energies(aenet_amd_mc *mc, double E_chain[], int *stat);
+/* out (may be NULL): nsamples*nchains energies of the last run */
+void aenet_amd_mc_samples(aenet_amd_mc *mc, double out[], long long *nsamples, int *stat);
+void aenet_amd_mc_state(aenet_amd_mc *mc, double coords[], int types[], int *stat);
+/* Decision record: max_steps >= 0 arms it for the next runs (steps
+ * counted from the start of each run; 0 disarms); max_steps < 0 copies it
+ * into out (may be NULL): per step and chain 4 doubles -- first-stage
+ * energy change, full-potential change (0 if not evaluated), stage-1
+ * outcome (1 passed, 0 rejected, -1 no move), accepted. */
+void aenet_amd_mc_trace(aenet_amd_mc *mc, long long max_steps, double out[], long long *nsteps,
+                        int *stat);
+/* first-stage atomic energies and unscaled descriptors (nsf per atom) */
+void aenet_amd_mc_first_stage_state(aenet_amd_mc *mc, double E_atoms[], double G[], int *nsf,
+                                    int *stat);
+
+/* Fit of a first-stage potential (radial-only Chebyshev, one hidden layer
+ * of at most 16 nodes, linear output) to reference energies and forces of
+ * ncells cells of natoms atoms (<= 256) with one lattice: d_coords
+ * (3*ncells*natoms), d_types, d_E_ref (atomic energies), d_F_ref
+ * (forces).  The cells d_sel[0 .. nsel-1] contribute; d_grad (wsize
+ * floats) += gradient of coefE/2 sum_i (E_i - E_ref_i)^2 + coefF/2 sum_i
+ * |F_i - F_ref_i|^2; d_loss[0], d_loss[1] += the two sums of squares.
+ * The force term is differentiated through the network's input gradient
+ * (forward tangent along the residual's pull-back, then reverse). */
+void aenet_amd_model_fit_cells(aenet_amd_model *model, int ncells, int natoms, const double latt[9],
+                               const double *d_coords, const int *d_types, const int *d_sel,
+                               int nsel, const double *d_E_ref, const double *d_F_ref,
+                               double coefE, double coefF, float *d_grad, double *d_loss,
+                               void *stream, int *stat);
+
 /* Text of the last error of the calling thread. */
 const char *aenet_amd_last_error(void);
 
