@@ -604,7 +604,7 @@ static void launch_bp(const ModelDev &md, const BatchDev &b, const NbDev &nb, co
                       BPLaunch P, hipStream_t stream) {
   PersistentShape ps;
   const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
-  if (!persistent_shape(wbytes, 4 * (size_t)bp_wave_words(P), P.natoms, ps)) {
+  if (!persistent_shape(wbytes, 4 * (size_t)bp_wave_words(P), P.natoms, 3, ps)) {
     raise_flag(b.flags, AENET_AMD_FLAG_NB_OVERFLOW, stream);
     return;
   }

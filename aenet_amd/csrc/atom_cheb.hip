@@ -74,7 +74,7 @@ __device__ inline WaveLds carve(float *base, const ChebLaunch &P) {
 // MODE 0: structures (compact lists, FP64 atomic force scatter)
 // MODE 1: explicit environments (per-neighbor forces out)
 template <int RN, int AN, int MODE, bool FORCES>
-__global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_cheb(ModelDev md, BatchDev b, NbDev nbl, OutDev o, EnvDev env,
+__global__ __launch_bounds__(64 * ATOM_MAX_WAVES, 4) void k_atom_cheb(ModelDev md, BatchDev b, NbDev nbl, OutDev o, EnvDev env,
                                                           ChebLaunch P) {
   static_assert(2 * (RN + AN) <= 64, "features of one chunk");
   extern __shared__ float lds[];
@@ -377,7 +377,7 @@ static void launch_one(const ModelDev &md, const BatchDev &b, const NbDev &nb, c
                        ChebLaunch P, hipStream_t stream) {
   PersistentShape ps;
   const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
-  if (!persistent_shape(wbytes, 4 * (size_t)wave_lds_words(P), P.natoms, ps)) {
+  if (!persistent_shape(wbytes, 4 * (size_t)wave_lds_words(P), P.natoms, 4, ps)) {
     // cannot happen within the capacity limits; refuse rather than overrun LDS
     raise_flag(b.flags, AENET_AMD_FLAG_NB_OVERFLOW, stream);
     return;

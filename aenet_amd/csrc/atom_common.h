@@ -216,26 +216,39 @@ __device__ __forceinline__ double atom_energy(const SpeciesDev &sp, float y) {
   return E + sp.E_atom;
 }
 
-// Persistent launch shape for an atom kernel whose workgroups hold wbytes
-// of shared data (the weights) plus wave_bytes per wavefront: as many
-// wavefronts per workgroup as LDS allows (up to ATOM_MAX_WAVES) and as many
-// workgroups as fit on the device at once.  Returns false when not even
-// one wavefront fits.
+// Persistent launch shape for an atom kernel whose workgroups may hold
+// wbytes of shared data (the network weights) plus wave_bytes per
+// wavefront.  simd_waves is the kernel's register-limited wavefronts per
+// SIMD.  The weights are staged in LDS only when that does not lower the
+// number of wavefronts resident per CU (otherwise they are read through
+// the caches); as many wavefronts per workgroup as LDS allows (up to
+// ATOM_MAX_WAVES) and as many workgroups as fit on the device at once.
+// Returns false when not even one wavefront fits.
 struct PersistentShape {
   int waves, grid;
   size_t lds;
   bool w_in_lds;
 };
 
-inline bool persistent_shape(size_t wbytes, size_t wave_bytes, int natoms, PersistentShape &ps) {
+inline int resident_waves(size_t wbytes, size_t wave_bytes, int simd_waves, int &waves_per_wg) {
   const DeviceLimits &dl = device_limits();
-  ps.w_in_lds = wbytes > 0 && wbytes + wave_bytes <= dl.lds_per_block && wbytes <= 64 * 1024;
-  if (!ps.w_in_lds) wbytes = 0;
-  if (wave_bytes > dl.lds_per_block - wbytes) return false;
-  ps.waves = (int)std::min<size_t>(ATOM_MAX_WAVES, (dl.lds_per_block - wbytes) / wave_bytes);
-  ps.lds = wbytes + (size_t)ps.waves * wave_bytes;
-  int per_cu = (int)(dl.lds_per_cu / ps.lds);
-  per_cu = std::max(1, std::min(per_cu, 32 / ps.waves));
+  if (wbytes + wave_bytes > dl.lds_per_block) return 0;
+  waves_per_wg = (int)std::min<size_t>(ATOM_MAX_WAVES, (dl.lds_per_block - wbytes) / wave_bytes);
+  const size_t wg_bytes = wbytes + (size_t)waves_per_wg * wave_bytes;
+  const int by_lds = (int)(dl.lds_per_cu / wg_bytes) * waves_per_wg;
+  return std::min(by_lds, 4 * simd_waves);
+}
+
+inline bool persistent_shape(size_t wbytes, size_t wave_bytes, int natoms, int simd_waves, PersistentShape &ps) {
+  const DeviceLimits &dl = device_limits();
+  int wg_plain = 0, wg_staged = 0;
+  const int r_plain = resident_waves(0, wave_bytes, simd_waves, wg_plain);
+  const int r_staged = (wbytes > 0 && wbytes <= 64 * 1024) ? resident_waves(wbytes, wave_bytes, simd_waves, wg_staged) : 0;
+  if (r_plain == 0 && r_staged == 0) return false;
+  ps.w_in_lds = r_staged > 0 && r_staged >= r_plain;
+  ps.waves = ps.w_in_lds ? wg_staged : wg_plain;
+  ps.lds = (ps.w_in_lds ? wbytes : 0) + (size_t)ps.waves * wave_bytes;
+  const int per_cu = std::max(1, (ps.w_in_lds ? r_staged : r_plain) / ps.waves);
   ps.grid = std::min((natoms + ps.waves - 1) / ps.waves, dl.cus * per_cu);
   ps.grid = std::max(ps.grid, 1);
   return true;

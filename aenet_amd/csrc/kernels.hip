@@ -37,9 +37,8 @@ __global__ void k_scale_f64(double *p, double a, long n) {
 // cell (cell_get, src/ext/lclist.f90:833-845)
 // ---------------------------------------------------------------------
 
-__global__ void k_frac_bin(BatchDev b, double *F) {
-  int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= b.natoms) return;
+// lattice coordinates and cell of atom g (returns the global cell id)
+__device__ __forceinline__ int frac_cell(const BatchDev &b, int g, double *F) {
   if (F) {
     F[3 * (size_t)g] = 0.0;
     F[3 * (size_t)g + 1] = 0.0;
@@ -96,7 +95,82 @@ __global__ void k_frac_bin(BatchDev b, double *F) {
   b.s[3 * (size_t)g + 1] = s[1];
   b.s[3 * (size_t)g + 2] = s[2];
   b.cellof[g] = cell;
-  atomicAdd(&b.cell_count[cell], 1);
+  return cell;
+}
+
+__global__ void k_frac_bin(BatchDev b, double *F) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= b.natoms) return;
+  atomicAdd(&b.cell_count[frac_cell(b, g, F)], 1);
+}
+
+// the whole binning of a small batch in one workgroup: counts in LDS,
+// exclusive scan, placement, and the order within each cell (descending
+// atom index, as the reference's linked lists) with the lattice
+// coordinates copied into that order
+constexpr int SMALL_ATOMS = 8192, SMALL_CELLS = 3072;
+__global__ __launch_bounds__(1024) void k_bin_small(BatchDev b, double *F) {
+  __shared__ int cnt[SMALL_CELLS + 1];
+  __shared__ int fill[SMALL_CELLS];
+  __shared__ int slot[SMALL_ATOMS];
+  __shared__ int wsum[16];
+  const int tid = threadIdx.x;
+  for (int c = tid; c < b.ncells; c += blockDim.x) {
+    cnt[c] = 0;
+    fill[c] = 0;
+  }
+  __syncthreads();
+  for (int g = tid; g < b.natoms; g += blockDim.x) atomicAdd(&cnt[frac_cell(b, g, F)], 1);
+  __syncthreads();
+  // exclusive scan of cnt -> cell_start (global) and cnt (LDS, in place)
+  int carry = 0;
+  for (int base = 0; base < b.ncells; base += 1024) {
+    const int i = base + tid;
+    const int v = i < b.ncells ? cnt[i] : 0;
+    const int x = wave_scan(v);
+    const int w = tid / WAVE;
+    if (lane_id() == WAVE - 1) wsum[w] = x;
+    __syncthreads();
+    if (tid < 16) {
+      int t = wsum[tid];
+      for (int o = 1; o < 16; o <<= 1) {
+        const int y = __shfl_up(t, o, 16);
+        if (tid >= o) t += y;
+      }
+      wsum[tid] = t;
+    }
+    __syncthreads();
+    const int off = carry + (w > 0 ? wsum[w - 1] : 0) + x - v;
+    const int tot = wsum[15];
+    __syncthreads();
+    if (i < b.ncells) {
+      cnt[i] = off;
+      b.cell_start[i] = off;
+    }
+    carry += tot;
+  }
+  if (tid == 0) {
+    cnt[b.ncells] = carry;
+    b.cell_start[b.ncells] = carry;
+  }
+  __syncthreads();
+  for (int g = tid; g < b.natoms; g += blockDim.x) {
+    const int c = b.cellof[g];
+    slot[cnt[c] + atomicAdd(&fill[c], 1)] = g;
+  }
+  __syncthreads();
+  for (int p = tid; p < b.natoms; p += blockDim.x) {
+    const int v = slot[p];
+    const int c = b.cellof[v];
+    const int start = cnt[c], n = cnt[c + 1] - start;
+    int rank = 0;
+    for (int k = 0; k < n; ++k) rank += slot[start + k] > v;
+    const int q = start + rank;
+    b.cell_atoms[q] = v;
+    b.sorted_s[3 * (size_t)q] = b.s[3 * (size_t)v];
+    b.sorted_s[3 * (size_t)q + 1] = b.s[3 * (size_t)v + 1];
+    b.sorted_s[3 * (size_t)q + 2] = b.s[3 * (size_t)v + 2];
+  }
 }
 
 // exclusive scan of the cell counts (one workgroup)
@@ -400,6 +474,10 @@ void launch_combos(const BatchDev &b, int *combo_count, int *combos, hipStream_t
 }
 
 void launch_binning(const BatchDev &b, int *tmp, double *F, hipStream_t s) {
+  if (b.natoms <= SMALL_ATOMS && b.ncells <= SMALL_CELLS) {
+    hipLaunchKernelGGL(k_bin_small, dim3(1), dim3(1024), 0, s, b, F);
+    return;
+  }
   // cell_count is zero on entry (cleared at allocation and by k_cell_sort)
   hipLaunchKernelGGL(k_frac_bin, dim3(nblk(b.natoms, 256)), dim3(256), 0, s, b, F);
   hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, s, b);

@@ -143,7 +143,7 @@ void launch_net_grad(const ModelDev &md, const NetGradArgs &a, int natoms, int n
   P.wsize = wsize;
   PersistentShape ps;
   const size_t gbytes = grad ? 4 * (size_t)((wsize + 3) / 4 * 4) : 0;
-  if (!persistent_shape(gbytes, 4 * (size_t)net_grad_wave_words(P), natoms, ps)) {
+  if (!persistent_shape(gbytes, 4 * (size_t)net_grad_wave_words(P), natoms, 4, ps)) {
     if (a.flags) raise_flag(a.flags, AENET_AMD_FLAG_NB_OVERFLOW, stream);
     return;
   }

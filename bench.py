@@ -22,7 +22,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-METRIC = "atom energy+force evals/s (1 GPU, FP32 vs FP64 ref)"
+METRIC = "atom energy+force evals/s (1 GPU, FP32 vs FP64 ref); DA-MC vs plain accepted moves/s"
 FP32_PEAK = 157.3e12      # MI355X dense FP32 vector, FLOP/s
 HBM_PEAK = 8.0e12         # B/s
 # algorithmic FP32 FLOPs (FMA = 2) per unit of the Chebyshev path with
@@ -208,6 +208,39 @@ def batched_bp(steps, warmup):
         ann.release()
 
 
+def mc_throughput(steps, burn):
+    """Accepted Monte Carlo moves per second, plain Metropolis with the
+    teacher potential against delayed acceptance with the trained
+    radial-only student as first stage: 512 chains of the 256-atom binary
+    FCC cell (a = 3.61 A), 600 K, 50 % swaps / 50 % displacements
+    U(+-0.075 A)^3, both from the same burned-in states."""
+    from aenet_amd import synth
+    from aenet_amd.mc import MonteCarlo
+    from aenet_amd.model import Model
+    d = os.path.join(ROOT, "tests", "golden", "potentials")
+    teacher = Model({t: os.path.join(d, "teacher.%s.nn" % t) for t in "AB"})
+    student = Model({t: os.path.join(d, "student_trained.%s.nn" % t) for t in "AB"})
+    cells = [synth.mc_cell(100 + c) for c in range(512)]
+    out = {"workload": "mc512x256: 512 chains x 256 atoms (4x4x4 FCC A/B, a=3.61 A), 600 K, 50% swaps, "
+                       "50% displacements U(+-0.075 A)^3; teacher Chebyshev 16/6.0 + 4/4.0 typespin 44-20-20-1; "
+                       "first stage radial-only Chebyshev 8/5.0 typespin 20-8-1 trained on teacher MC data",
+           "steps": steps, "burn_in": burn}
+    for name, first, delayed in (("plain", None, False), ("delayed", student, True)):
+        mc = MonteCarlo(teacher, cells, first=first, seed=7)
+        mc.set(T=600.0)
+        mc.run(burn)
+        mc.stats(reset=True)
+        t = mc.run(steps, delayed=delayed)
+        st = mc.stats()
+        out[name] = {"accepted_moves_per_s": st["accepted"] / t, "proposals_per_s": st["proposed"] / t,
+                     "acceptance": st["accepted"] / st["proposed"],
+                     "full_potential_fraction": st["full_evaluations"] / st["proposed"],
+                     "ms_per_step": t / steps * 1e3}
+        del mc
+    out["delayed_over_plain"] = out["delayed"]["accepted_moves_per_s"] / out["plain"]["accepted_moves_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +252,7 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=1, help="sample = every (procs*stride)th atom")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched Behler-Parrinello measurement")
+    ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the prediction path runs on a single MI355X and this build "
@@ -316,6 +350,8 @@ def main():
     ann.release()
     if not args.no_batch:
         out["batched_bp"] = batched_bp(max(5, min(args.steps, 20)), 2)
+    if not args.no_mc:
+        out["mc"] = mc_throughput(3000, 1000)
     print(json.dumps(out))
 
 
