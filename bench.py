@@ -253,7 +253,20 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched Behler-Parrinello measurement")
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo measurement")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer 5 times (reference kernel for the "
+                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 5}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the prediction path runs on a single MI355X and this build "
               "has no multi-GPU mode (--gpus %d requested)" % args.gpus, file=sys.stderr)
@@ -306,6 +319,11 @@ def main():
     n_p, n_t = pair_triplet_counts(db)
     flops = n_p * FLOP_PAIR + n_t * FLOP_TRIPLET + natoms * FLOP_ATOM
     achieved = flops / k_avg_s
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_atom_cheb_alloy4000_traffic.json")
+    if args.replicas == 1 and os.path.exists(tpath):
+        t = json.load(open(tpath))["kernels"]["k_atom_cheb<19,7,0,true>"]
+        traffic = t["read_bytes"] + t["write_bytes"]
 
     out = {
         "metric": METRIC,
@@ -336,7 +354,9 @@ def main():
             "peak": FP32_PEAK / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, scaled by a 256 MiB copy "
+                              "(profiles/k_atom_cheb_alloy4000_traffic.md)" if traffic else None,
             "kernel_ms": k_avg_s * 1e3,
             "flop_per_launch": flops,
             "pairs": n_p,
