@@ -89,6 +89,7 @@ struct AtomSizes {
   int nsf_max;  // longest descriptor
   int wsize;    // total network weights
   int nl_max;   // most network layers (input layer included)
+  int hmax;     // widest non-input layer
   // optional subset of central atoms (structures mode): atom indices
   // active[0 .. *nactive-1]; natoms is then the largest possible count
   const int *active;

@@ -67,12 +67,13 @@ struct BPLaunch {
   int wsize;     // total weights
   int w_in_lds;  // weights staged in LDS
   int waves;     // wavefronts per workgroup
+  int hs;        // network layer stride
   const int *active;   // optional subset of central atoms (mode 0)
   const int *nactive;
 };
 
 __host__ __device__ inline int bp_wave_words(const BPLaunch &P) {
-  return 10 * P.cap + QCAP + 16 + net_lds_words(P.nsf, P.nl);
+  return 10 * P.cap + QCAP + 16 + net_lds_words(P.nsf, P.nl, P.hs);
 }
 
 __device__ inline BPWave bp_carve(float *base, const BPLaunch &P) {
@@ -86,7 +87,7 @@ __device__ inline BPWave bp_carve(float *base, const BPLaunch &P) {
   w.fs = reinterpret_cast<float *>(w.bidx + cap);
   w.queue = reinterpret_cast<int *>(w.fs + 3 * cap);
   w.bstart = w.queue + QCAP;
-  w.net = carve_net(reinterpret_cast<float *>(w.bstart + 16), P.nsf, P.nl);
+  w.net = carve_net(reinterpret_cast<float *>(w.bstart + 16), P.nsf, P.nl, P.hs);
   return w;
 }
 
@@ -621,6 +622,7 @@ void launch_atom_bp(int mode, bool forces, const AtomSizes &z, const ModelDev &m
   P.cap = z.cap;
   P.nsf = z.nsf_max;
   P.nl = std::max(z.nl_max, 2);
+  P.hs = net_stride(z.hmax);
   P.wsize = z.wsize;
   P.w_in_lds = 0;
   P.waves = 1;

@@ -30,11 +30,16 @@ namespace aenet_amd {
 
 namespace {
 constexpr float EPS_R = 1.0e-12f;
+constexpr int AUXN = 64;  // angular neighbors whose per-neighbor terms are tabulated in LDS
+// wavefronts per workgroup: up to a CU's worth, so that one staged copy of
+// the network weights serves every wavefront of the CU
+constexpr int CHEB_MAX_WAVES = 16;
 }  // namespace
 
 // Per-wavefront LDS region, laid out after the weights in dynamic shared
 // memory; cap = neighbor capacity.
 struct WaveLds {
+  float4 *aux; // [AUXN] per angular neighbor: 1/r, fc, dfc/dr, typespin
   float *R;    // [3*cap]
   float *r;    // [cap]
   int *t;      // [cap] neighbor types
@@ -52,29 +57,33 @@ struct ChebLaunch {
   int wsize;     // total weights
   int w_in_lds;  // weights staged in LDS
   int waves;     // wavefronts per workgroup
+  int hs;        // network layer stride
   const int *active;   // optional subset of central atoms (mode 0)
   const int *nactive;
 };
 
-__host__ __device__ inline int wave_lds_words(const ChebLaunch &P) { return 10 * P.cap + net_lds_words(P.nsf, P.nl); }
+__host__ __device__ inline int wave_lds_words(const ChebLaunch &P) {
+  return (4 * AUXN + 10 * P.cap + net_lds_words(P.nsf, P.nl, P.hs) + 3) / 4 * 4;
+}
 
 __device__ inline WaveLds carve(float *base, const ChebLaunch &P) {
   const int cap = P.cap;
   WaveLds w;
-  w.R = base;
+  w.aux = reinterpret_cast<float4 *>(base);
+  w.R = base + 4 * AUXN;
   w.r = w.R + 3 * cap;
   w.t = reinterpret_cast<int *>(w.r + cap);
   w.j = w.t + cap;
   w.ang = w.j + cap;
   w.fs = reinterpret_cast<float *>(w.ang + cap);
-  w.net = carve_net(w.fs + 3 * cap, P.nsf, P.nl);
+  w.net = carve_net(w.fs + 3 * cap, P.nsf, P.nl, P.hs);
   return w;
 }
 
 // MODE 0: structures (compact lists, FP64 atomic force scatter)
 // MODE 1: explicit environments (per-neighbor forces out)
 template <int RN, int AN, int MODE, bool FORCES>
-__global__ __launch_bounds__(64 * ATOM_MAX_WAVES, 4) void k_atom_cheb(ModelDev md, BatchDev b, NbDev nbl, OutDev o, EnvDev env,
+__global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev md, BatchDev b, NbDev nbl, OutDev o, EnvDev env,
                                                           ChebLaunch P) {
   static_assert(2 * (RN + AN) <= 64, "features of one chunk");
   extern __shared__ float lds[];
@@ -186,19 +195,30 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES, 4) void k_atom_cheb(ModelDev m
       na += popc64(m);
     }
     wave_sync();
+    // per angular neighbor: 1/r, cutoff and its derivative, typespin --
+    // tabulated once per atom for both triplet passes
+    const bool use_aux = na <= AUXN;
+    auto ang_terms = [&](int q) -> float4 {
+      const float r = S.r[q];
+      float fc, dfc;
+      cutoff(r, aRc, inv_aRc, fc, dfc);
+      return make_float4(1.0f / r, fc, dfc, multi ? sp.spin[S.t[q] - 1] : 0.0f);
+    };
+    if (use_aux) {
+      if (lane < na) S.aux[lane] = ang_terms(S.ang[lane]);
+      wave_sync();
+    }
+    auto ang_get = [&](int a, int q) -> float4 { return use_aux ? S.aux[a] : ang_terms(q); };
     const int ntrip = na * (na - 1) / 2;
     for (int p = lane; p < ntrip; p += WAVE) {
       int a, c;
       tri_decode(p, na, a, c);
       int qj = S.ang[a], qk = S.ang[c];
-      float rj = S.r[qj], rk = S.r[qk];
-      float cosv = (S.R[3 * qj] * S.R[3 * qk] + S.R[3 * qj + 1] * S.R[3 * qk + 1] + S.R[3 * qj + 2] * S.R[3 * qk + 2]) /
-                   (rj * rk);
-      float fcj, dfcj, fck, dfck;
-      cutoff(rj, aRc, inv_aRc, fcj, dfcj);
-      cutoff(rk, aRc, inv_aRc, fck, dfck);
-      float w = fcj * fck;
-      float sw = sp.spin[S.t[qj] - 1] * sp.spin[S.t[qk] - 1] * w;
+      const float4 Aj = ang_get(a, qj), Ak = ang_get(c, qk);
+      float cosv = (S.R[3 * qj] * S.R[3 * qk] + S.R[3 * qj + 1] * S.R[3 * qk + 1] + S.R[3 * qj + 2] * S.R[3 * qk + 2]) *
+                   (Aj.x * Ak.x);
+      float w = Aj.y * Ak.y;
+      float sw = Aj.w * Ak.w * w;
       float Tm = 1.0f, T = cosv;
       acc[RN] += w;
       acc[2 * RN + AN] += sw;
@@ -289,31 +309,35 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES, 4) void k_atom_cheb(ModelDev m
       fs[3 * q + 2] = gz;
     }
     wave_sync();
+    // dE/dG of the angular terms in registers: plain (p) and typespin (s)
+    float cap_[AN], cas_[AN];
+#pragma unroll
+    for (int k = 0; k < AN; ++k) {
+      cap_[k] = k < aN ? ca1[k] : 0.0f;
+      cas_[k] = (multi && k < aN) ? ca2[k] : 0.0f;
+    }
     for (int p = lane; p < ntrip; p += WAVE) {
       int a, c;
       tri_decode(p, na, a, c);
       int qj = S.ang[a], qk = S.ang[c];
-      float rj = S.r[qj], rk = S.r[qk];
+      const float4 Aj = ang_get(a, qj), Ak = ang_get(c, qk);
+      const float ij = Aj.x, ik = Ak.x;
       float Rj[3] = {S.R[3 * qj], S.R[3 * qj + 1], S.R[3 * qj + 2]};
       float Rk[3] = {S.R[3 * qk], S.R[3 * qk + 1], S.R[3 * qk + 2]};
-      float ijk = 1.0f / (rj * rk);
+      float ijk = ij * ik;
       float cosv = (Rj[0] * Rk[0] + Rj[1] * Rk[1] + Rj[2] * Rk[2]) * ijk;
-      float fcj, dfcj, fck, dfck;
-      cutoff(rj, aRc, inv_aRc, fcj, dfcj);
-      cutoff(rk, aRc, inv_aRc, fck, dfck);
+      const float fcj = Aj.y, dfcj = Aj.z, fck = Ak.y, dfck = Ak.z;
       float w = fcj * fck;
-      float ss = multi ? sp.spin[S.t[qj] - 1] * sp.spin[S.t[qk] - 1] : 0.0f;
-      // A = sum b_n T_n(cos), B = sum b_n T'_n(cos)
-      float A = ca1[0] + ss * ca2[0];
-      float Bd = 0.0f;
+      float ss = Aj.w * Ak.w;
+      // A = sum_n (p_n + ss s_n) T_n(cos), B = sum_n (p_n + ss s_n) T'_n(cos)
+      float Ap = cap_[0], As = cas_[0], Bp = 0.0f, Bs = 0.0f;
       float Tm = 1.0f, T = cosv, Um = 0.0f, U = 1.0f;
 #pragma unroll
       for (int k = 1; k < AN; ++k) {
-        if (k < aN) {
-          float bk = ca1[k] + ss * ca2[k];
-          A += bk * T;
-          Bd += bk * (float)k * U;
-        }
+        Ap += cap_[k] * T;
+        As += cas_[k] * T;
+        Bp += cap_[k] * (float)k * U;
+        Bs += cas_[k] * (float)k * U;
         float Tn = 2.0f * cosv * T - Tm;
         float Un = 2.0f * cosv * U - Um;
         Tm = T;
@@ -321,9 +345,11 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES, 4) void k_atom_cheb(ModelDev m
         Um = U;
         U = Un;
       }
+      const float A = Ap + ss * As;
+      const float Bd = Bp + ss * Bs;
       float wB = w * Bd;
-      float aj = A * dfcj * fck / rj, ak = A * fcj * dfck / rk;
-      float cj = cosv / (rj * rj), ck = cosv / (rk * rk);
+      float aj = A * dfcj * fck * ij, ak = A * fcj * dfck * ik;
+      float cj = cosv * ij * ij, ck = cosv * ik * ik;
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
         float gj = aj * Rj[x] + wB * (Rk[x] * ijk - cj * Rj[x]);
@@ -377,7 +403,7 @@ static void launch_one(const ModelDev &md, const BatchDev &b, const NbDev &nb, c
                        ChebLaunch P, hipStream_t stream) {
   PersistentShape ps;
   const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
-  if (!persistent_shape(wbytes, 4 * (size_t)wave_lds_words(P), P.natoms, 4, ps)) {
+  if (!persistent_shape(wbytes, 4 * (size_t)wave_lds_words(P), P.natoms, 4, ps, CHEB_MAX_WAVES)) {
     // cannot happen within the capacity limits; refuse rather than overrun LDS
     raise_flag(b.flags, AENET_AMD_FLAG_NB_OVERFLOW, stream);
     return;
@@ -412,6 +438,7 @@ void launch_atom_cheb(int rN, int aN, int mode, bool forces, const AtomSizes &z,
   P.cap = z.cap;
   P.nsf = z.nsf_max;
   P.nl = std::max(z.nl_max, 2);
+  P.hs = net_stride(z.hmax);
   P.wsize = z.wsize;
   P.w_in_lds = 0;
   P.waves = 1;

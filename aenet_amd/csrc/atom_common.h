@@ -14,7 +14,7 @@
 
 namespace aenet_amd {
 
-constexpr int ATOM_MAX_WAVES = 4;  // wavefronts per workgroup (fewer when LDS is short)
+constexpr int ATOM_MAX_WAVES = 4;  // wavefronts per workgroup of most atom kernels (fewer when LDS is short)
 
 // orders the LDS accesses of one wavefront (lanes exchange data through
 // LDS between phases of one atom)
@@ -126,26 +126,31 @@ __device__ __forceinline__ double reduce_scatter(float (&v)[V]) {
 struct NetLds {
   float *gp;    // [nsf] normalized descriptor
   float *coef;  // [nsf] E_scale * dy/dG' * norm scale = dE/dG (unnormalized G)
-  float *d0;    // [dmax] back-propagation buffers, dmax = max(nsf, 64)
+  float *d0;    // [dmax] back-propagation buffers, dmax = max(nsf, hs)
   float *d1;    // [dmax]
-  float *act;   // [nl*64] outputs of layer l at act[64*l]
-  float *dact;  // [nl*64] activation derivatives
+  float *act;   // [nl*hs] outputs of layer l at act[hs*l]
+  float *dact;  // [nl*hs] activation derivatives
+  int hs;       // layer stride: the widest non-input layer, rounded up to 32
 };
 
-__host__ __device__ inline int net_dmax(int nsf) { return nsf > AENET_AMD_MAX_HIDDEN ? nsf : AENET_AMD_MAX_HIDDEN; }
-__host__ __device__ inline int net_lds_words(int nsf, int nl) {
-  return 2 * nsf + 2 * net_dmax(nsf) + 2 * nl * AENET_AMD_MAX_HIDDEN;
+__host__ __device__ inline int net_dmax(int nsf, int hs) { return nsf > hs ? nsf : hs; }
+__host__ __device__ inline int net_lds_words(int nsf, int nl, int hs) {
+  return 2 * nsf + 2 * net_dmax(nsf, hs) + 2 * nl * hs;
 }
-__device__ inline NetLds carve_net(float *p, int nsf, int nl) {
+__device__ inline NetLds carve_net(float *p, int nsf, int nl, int hs) {
   NetLds n;
+  n.hs = hs;
   n.gp = p;
   n.coef = n.gp + nsf;
   n.d0 = n.coef + nsf;
-  n.d1 = n.d0 + net_dmax(nsf);
-  n.act = n.d1 + net_dmax(nsf);
-  n.dact = n.act + nl * AENET_AMD_MAX_HIDDEN;
+  n.d1 = n.d0 + net_dmax(nsf, hs);
+  n.act = n.d1 + net_dmax(nsf, hs);
+  n.dact = n.act + nl * hs;
   return n;
 }
+
+// layer stride for a model whose widest non-input layer has hmax nodes
+__host__ __device__ inline int net_stride(int hmax) { return hmax <= 32 ? 32 : AENET_AMD_MAX_HIDDEN; }
 
 // Feed-forward network of one species on the normalized descriptor in
 // N.gp: ff_eval (src/ext/feedforward.f90:586-655), lanes = nodes of a
@@ -172,17 +177,17 @@ __device__ inline float network(const ModelDev &md, const SpeciesDev &sp, const 
       for (; c < n1; ++c) z0 += Wl[c * n2 + lane] * in[c];
       float y, dy;
       activate(sp.fa[l + 1], (z0 + z1) + (z2 + z3), y, dy);
-      N.act[(l + 1) * AENET_AMD_MAX_HIDDEN + lane] = y;
-      N.dact[(l + 1) * AENET_AMD_MAX_HIDDEN + lane] = dy;
+      N.act[(l + 1) * N.hs + lane] = y;
+      N.dact[(l + 1) * N.hs + lane] = dy;
     }
     wave_sync();
-    in = N.act + (l + 1) * AENET_AMD_MAX_HIDDEN;
+    in = N.act + (l + 1) * N.hs;
   }
-  const float yout = N.act[(nl - 1) * AENET_AMD_MAX_HIDDEN];
+  const float yout = N.act[(nl - 1) * N.hs];
   if (!grad) return yout;
   float *dcur = N.d0;
   float *dnext = N.d1;
-  if (lane == 0) dcur[0] = N.dact[(nl - 1) * AENET_AMD_MAX_HIDDEN];
+  if (lane == 0) dcur[0] = N.dact[(nl - 1) * N.hs];
   wave_sync();
   for (int l = nl - 2; l >= 0; --l) {
     const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
@@ -197,7 +202,7 @@ __device__ inline float network(const ModelDev &md, const SpeciesDev &sp, const 
       }
       if (hh < n2) s0 += row[hh] * dcur[hh];
       float s = s0 + s1;
-      dnext[c] = l > 0 ? s * N.dact[l * AENET_AMD_MAX_HIDDEN + c] : s;
+      dnext[c] = l > 0 ? s * N.dact[l * N.hs + c] : s;
     }
     wave_sync();
     float *t = dcur;
@@ -230,20 +235,23 @@ struct PersistentShape {
   bool w_in_lds;
 };
 
-inline int resident_waves(size_t wbytes, size_t wave_bytes, int simd_waves, int &waves_per_wg) {
+inline int resident_waves(size_t wbytes, size_t wave_bytes, int simd_waves, int max_waves, int &waves_per_wg) {
   const DeviceLimits &dl = device_limits();
   if (wbytes + wave_bytes > dl.lds_per_block) return 0;
-  waves_per_wg = (int)std::min<size_t>(ATOM_MAX_WAVES, (dl.lds_per_block - wbytes) / wave_bytes);
+  waves_per_wg = (int)std::min<size_t>(max_waves, (dl.lds_per_block - wbytes) / wave_bytes);
   const size_t wg_bytes = wbytes + (size_t)waves_per_wg * wave_bytes;
   const int by_lds = (int)(dl.lds_per_cu / wg_bytes) * waves_per_wg;
   return std::min(by_lds, 4 * simd_waves);
 }
 
-inline bool persistent_shape(size_t wbytes, size_t wave_bytes, int natoms, int simd_waves, PersistentShape &ps) {
+inline bool persistent_shape(size_t wbytes, size_t wave_bytes, int natoms, int simd_waves, PersistentShape &ps,
+                             int max_waves = ATOM_MAX_WAVES) {
   const DeviceLimits &dl = device_limits();
   int wg_plain = 0, wg_staged = 0;
-  const int r_plain = resident_waves(0, wave_bytes, simd_waves, wg_plain);
-  const int r_staged = (wbytes > 0 && wbytes <= 64 * 1024) ? resident_waves(wbytes, wave_bytes, simd_waves, wg_staged) : 0;
+  // without staging, workgroups of at most ATOM_MAX_WAVES suffice
+  const int r_plain = resident_waves(0, wave_bytes, simd_waves, std::min(max_waves, ATOM_MAX_WAVES), wg_plain);
+  const int r_staged =
+      (wbytes > 0 && wbytes <= 64 * 1024) ? resident_waves(wbytes, wave_bytes, simd_waves, max_waves, wg_staged) : 0;
   if (r_plain == 0 && r_staged == 0) return false;
   ps.w_in_lds = r_staged > 0 && r_staged >= r_plain;
   ps.waves = ps.w_in_lds ? wg_staged : wg_plain;

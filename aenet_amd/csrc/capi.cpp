@@ -992,7 +992,7 @@ void aenet_amd_model_net_grad(aenet_amd_model *model, int natoms, const double *
   a.loss = d_loss;
   a.flags = model ? model->flags.p : nullptr;
   hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
-  launch_net_grad(dm->view(), a, natoms, dm->nsf_max, dm->nl_max, dm->wsize, d_grad != nullptr, s);
+  launch_net_grad(dm->view(), a, natoms, dm->nsf_max, dm->nl_max, dm->wsize, d_grad != nullptr, s, dm->hmax);
   *stat = hipGetLastError() == hipSuccess ? AENET_STATUS_OK : set_err(AENET_STATUS_ERR_DEVICE, "launch failed");
 }
 

@@ -19,7 +19,10 @@ int DeviceModel::upload(const PotSet &ps, hipStream_t s) {
   wsize = (int)ps.W.size();
   rc_max = ps.Rc_max;
   nl_max = 0;
-  for (const SpeciesDev &sd : ps.species) nl_max = std::max(nl_max, sd.nlayers);
+  for (const SpeciesDev &sd : ps.species) {
+    nl_max = std::max(nl_max, sd.nlayers);
+    for (int l = 1; l < sd.nlayers; ++l) hmax = std::max(hmax, sd.nnodes[l]);
+  }
   if (species.upload(ps.species.data(), ps.species.size(), s)) return AENET_STATUS_ERR_DEVICE;
   if (W.upload(ps.W.data(), ps.W.size(), s)) return AENET_STATUS_ERR_DEVICE;
   if (avg.upload(ps.norm_avg.data(), ps.norm_avg.size(), s)) return AENET_STATUS_ERR_DEVICE;
@@ -43,11 +46,47 @@ ModelDev DeviceModel::view() const {
   return m;
 }
 
+// fraction of a periodic cell's volume the atoms occupy: per lattice
+// direction one minus the widest empty gap between the atoms' fractional
+// coordinates (a slab's vacuum, wherever it sits in the cell)
+static double occupied_fraction(const double *L, int n, const double *cart) {
+  // L: column-major lattice (column k = vector k); fractional = L^-1 x
+  const double det = L[0] * (L[4] * L[8] - L[7] * L[5]) - L[3] * (L[1] * L[8] - L[7] * L[2]) +
+                     L[6] * (L[1] * L[5] - L[4] * L[2]);
+  if (n < 2 || !(std::fabs(det) > 1e-300)) return 1.0;
+  double inv[9];
+  inv[0] = (L[4] * L[8] - L[7] * L[5]) / det;
+  inv[3] = (L[6] * L[5] - L[3] * L[8]) / det;
+  inv[6] = (L[3] * L[7] - L[6] * L[4]) / det;
+  inv[1] = (L[7] * L[2] - L[1] * L[8]) / det;
+  inv[4] = (L[0] * L[8] - L[6] * L[2]) / det;
+  inv[7] = (L[6] * L[1] - L[0] * L[7]) / det;
+  inv[2] = (L[1] * L[5] - L[4] * L[2]) / det;
+  inv[5] = (L[3] * L[2] - L[0] * L[5]) / det;
+  inv[8] = (L[0] * L[4] - L[3] * L[1]) / det;
+  double occ = 1.0;
+  std::vector<double> f(n);
+  for (int k = 0; k < 3; ++k) {
+    for (int i = 0; i < n; ++i) {
+      double v = inv[k] * cart[3 * i] + inv[3 + k] * cart[3 * i + 1] + inv[6 + k] * cart[3 * i + 2];
+      f[i] = v - std::floor(v);
+    }
+    std::sort(f.begin(), f.end());
+    double gap = 1.0 - (f[n - 1] - f[0]);
+    for (int i = 1; i < n; ++i) gap = std::max(gap, f[i] - f[i - 1]);
+    occ *= std::max(0.05, 1.0 - gap);
+  }
+  return occ;
+}
+
 int Batch::create(const PotSet &ps, int nstruct, const int *natoms, const double *latt, const int *pbc,
                   const double *coords, std::string &err) {
   rc_max_ = ps.Rc_max;
   int st = build_plan(ps.Rc_max, nstruct, natoms, latt, pbc, coords, plan_, err);
   if (st) return st;
+  occ_.assign(nstruct, 1.0);
+  for (int s = 0, a0 = 0; s < nstruct; a0 += natoms[s], ++s)
+    if (pbc[s] && coords) occ_[s] = occupied_fraction(plan_.st[s].L, natoms[s], coords + 3 * (size_t)a0);
   return allocate(err);
 }
 
@@ -92,14 +131,16 @@ int Batch::allocate(std::string &err) {
   // neighbor capacity from the densest structure: atoms within the list
   // radius at that density, with headroom
   double est = 0.0;
-  for (auto &d : p.st) {
+  for (size_t si = 0; si < p.st.size(); ++si) {
+    const StructDev &d = p.st[si];
     double V = std::fabs(d.L[0] * (d.L[4] * d.L[8] - d.L[5] * d.L[7]) - d.L[3] * (d.L[1] * d.L[8] - d.L[2] * d.L[7]) +
                          d.L[6] * (d.L[1] * d.L[5] - d.L[2] * d.L[4]));
+    if (si < occ_.size()) V *= occ_[si];
     double rho = V > 0.0 ? d.natoms / V : 0.0;
     est = std::max(est, rho * 4.18879020478639 * d.rc_list * d.rc_list * d.rc_list + d.nself);
   }
-  int cap = (int)(1.25 * est) + 32;
-  cap = std::max(64, std::min(AENET_AMD_NB_CAP, (cap + 31) / 32 * 32));
+  int cap = (int)(1.2 * est) + 16;
+  cap = std::max(64, std::min(AENET_AMD_NB_CAP, (cap + 15) / 16 * 16));
   int st = set_nb_cap(cap);
   if (st) {
     err = "device allocation failed";
@@ -246,7 +287,7 @@ int Batch::vjp(const DeviceModel &dm, const double *d_coords, const int *d_types
     a.upstream = d_g;
     a.grad = d_gW;
     a.flags = flags_.p;
-    launch_net_grad(md, a, b.natoms, dm.nsf_max, dm.nl_max, dm.wsize, true, stream);
+    launch_net_grad(md, a, b.natoms, dm.nsf_max, dm.nl_max, dm.wsize, true, stream, dm.hmax);
   }
   return hipGetLastError() == hipSuccess ? AENET_STATUS_OK : AENET_STATUS_ERR_DEVICE;
 }

@@ -49,10 +49,10 @@ struct DeviceModel {
   DBuf<double> avg, scale;
   DBuf<BPDev> bp;
   DBuf<BPFunc> bpf;
-  int ntypes = 0, desc = 0, max_rN = 0, max_aN = 0, nsf_max = 0, wsize = 0, nl_max = 0;
+  int ntypes = 0, desc = 0, max_rN = 0, max_aN = 0, nsf_max = 0, wsize = 0, nl_max = 0, hmax = 1;
   double rc_max = 0.0;
   AtomSizes sizes(int natoms, int cap, const int *active = nullptr, const int *nactive = nullptr) const {
-    return AtomSizes{natoms, cap, nsf_max, wsize, nl_max, active, nactive};
+    return AtomSizes{natoms, cap, nsf_max, wsize, nl_max, hmax, active, nactive};
   }
   int upload(const PotSet &ps, hipStream_t s);
   ModelDev view() const;
@@ -112,6 +112,7 @@ class Batch {
   int allocate(std::string &err);
   PlanHost plan_;
   double rc_max_ = 0.0;
+  std::vector<double> occ_;  // occupied volume fraction of each structure (density estimate)
   DBuf<StructDev> st_;
   DBuf<int> atom_struct_, cell_struct_, cell_ijk_, Tvec_, Cvec_, self_t_;
   DBuf<double> self_d_;

@@ -43,8 +43,9 @@ void launch_atom_cheb(int rN, int aN, int mode, bool forces, const AtomSizes &z,
                       hipStream_t stream);
 
 // train.hip
+// hmax: the widest non-input layer of the model
 void launch_net_grad(const ModelDev &md, const NetGradArgs &a, int natoms, int nsf_max, int nl_max, int wsize,
-                     bool grad, hipStream_t stream);
+                     bool grad, hipStream_t stream, int hmax = AENET_AMD_MAX_HIDDEN);
 void launch_adam(float *W, const float *g, float *m, float *v, int n, int t, float lr, float b1, float b2,
                  float eps, hipStream_t stream);
 

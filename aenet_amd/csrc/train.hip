@@ -30,9 +30,10 @@ struct NetGradLaunch {
   int wsize;
   int grad_in_lds;
   int waves;
+  int hs;        // network layer stride
 };
 
-__host__ __device__ inline int net_grad_wave_words(const NetGradLaunch &P) { return net_lds_words(P.nsf, P.nl); }
+__host__ __device__ inline int net_grad_wave_words(const NetGradLaunch &P) { return net_lds_words(P.nsf, P.nl, P.hs); }
 
 template <bool GRAD>
 __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_net_grad(ModelDev md, NetGradArgs a, NetGradLaunch P) {
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_net_grad(ModelDev md, N
     for (int k = threadIdx.x; k < P.wsize; k += 64 * nw) gacc[k] = 0.0f;
     __syncthreads();
   }
-  NetLds N = carve_net(lds + gw + wave * net_grad_wave_words(P), P.nsf, P.nl);
+  NetLds N = carve_net(lds + gw + wave * net_grad_wave_words(P), P.nsf, P.nl, P.hs);
   for (int i = blockIdx.x * nw + wave; i < P.natoms; i += gridDim.x * nw) {
     wave_sync();
     const int ti = uniform(a.types[i]);
@@ -76,12 +77,12 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_net_grad(ModelDev md, N
     const int nl = sp.nlayers;
     float *dcur = N.d0;
     float *dnext = N.d1;
-    if (lane == 0) dcur[0] = N.dact[(nl - 1) * AENET_AMD_MAX_HIDDEN];
+    if (lane == 0) dcur[0] = N.dact[(nl - 1) * N.hs];
     wave_sync();
     const float s = (float)(g * sp.E_scale);
     for (int l = nl - 2; l >= 0; --l) {
       const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
-      const float *in = l == 0 ? N.gp : N.act + l * AENET_AMD_MAX_HIDDEN;
+      const float *in = l == 0 ? N.gp : N.act + l * N.hs;
       const int nblk = n2 * (n1 + 1);
       const int w0 = sp.w_off[l];
       // dE/dW_l[h + c*n2] = s * dcur[h] * in[c] (in[n1] = 1: bias)
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_net_grad(ModelDev md, N
           const float *row = Wl + c * n2;
           float acc = 0.0f;
           for (int h = 0; h < n2; ++h) acc += row[h] * dcur[h];
-          dnext[c] = acc * N.dact[l * AENET_AMD_MAX_HIDDEN + c];
+          dnext[c] = acc * N.dact[l * N.hs + c];
         }
       }
       wave_sync();
@@ -134,9 +135,10 @@ __global__ void k_adam(float *W, const float *g, float *m, float *v, int n, floa
 }
 
 void launch_net_grad(const ModelDev &md, const NetGradArgs &a, int natoms, int nsf_max, int nl_max, int wsize,
-                     bool grad, hipStream_t stream) {
+                     bool grad, hipStream_t stream, int hmax) {
   if (natoms <= 0) return;
   NetGradLaunch P;
+  P.hs = net_stride(hmax);
   P.natoms = natoms;
   P.nsf = nsf_max;
   P.nl = std::max(nl_max, 2);
