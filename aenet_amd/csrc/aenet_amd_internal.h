@@ -144,6 +144,7 @@ struct BatchDev {
   const int *combo_count;     // [ncells] (cell, image) combinations of each home cell
   const int *combos;          // [2*ncells*combo_cap] pairs (cell, image rank), visiting order
   int combo_cap;
+  int kept_cap;               // largest combination count of a home cell (search LDS size)
   const int *Tvec;            // all lattice translations
   const int *Cvec;            // all stencil vectors
   const int *self_t;          // self-image ranks

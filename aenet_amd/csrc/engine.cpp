@@ -153,6 +153,17 @@ int Batch::allocate(std::string &err) {
   launch_fill_i32(cell_fill_.p, 0, p.ncells, s);
   // the (cell, image) combinations depend on the lattice only
   launch_combos(view(nullptr, nullptr), combo_count_.p, combos_.p, s);
+  {
+    // the search's LDS holds the largest combination count of a home cell
+    std::vector<int> cc(p.ncells);
+    if (hipMemcpy(cc.data(), combo_count_.p, p.ncells * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) {
+      err = "device error";
+      return AENET_STATUS_ERR_DEVICE;
+    }
+    int mx = 1;
+    for (int v : cc) mx = std::max(mx, v);
+    kept_cap_ = std::min(AENET_AMD_KEPT_CAP, (mx + 15) / 16 * 16);
+  }
   if (hipDeviceSynchronize() != hipSuccess) {
     err = "device error";
     return AENET_STATUS_ERR_DEVICE;
@@ -202,6 +213,7 @@ BatchDev Batch::view(const double *d_coords, const int *d_types) const {
   b.combo_count = combo_count_.p;
   b.combos = combos_.p;
   b.combo_cap = combo_cap_;
+  b.kept_cap = kept_cap_;
   b.Tvec = Tvec_.p;
   b.Cvec = Cvec_.p;
   b.self_t = self_t_.p;

@@ -120,6 +120,7 @@ class Batch {
   DBuf<int> cellof_, cell_count_, cell_fill_, cell_start_, cell_atoms_, cell_tmp_, combo_count_, combos_, flags_;
   DBuf<double> cell_ctr_, sorted_s_;
   int combo_cap_ = 0;
+  int kept_cap_ = AENET_AMD_KEPT_CAP;
   // compact neighbor lists
   int nb_cap_ = 0;
   DBuf<int> nb_count_, nb_j_;

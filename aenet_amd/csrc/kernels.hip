@@ -365,7 +365,8 @@ struct CompactSink {
 };
 
 __global__ __launch_bounds__(64) void k_search(BatchDev b, NbDev nb, const int *active, const int *nactive) {
-  __shared__ SearchLds W;
+  extern __shared__ int sdyn[];
+  SearchLds W = carve_search(sdyn, b.kept_cap);
   __shared__ int ovf;
   int g = blockIdx.x;
   if (active) {
@@ -409,7 +410,8 @@ struct ListSink {
 };
 
 __global__ __launch_bounds__(64) void k_neighbor_lists(BatchDev b, OutDev o) {
-  __shared__ SearchLds W;
+  extern __shared__ int sdyn[];
+  SearchLds W = carve_search(sdyn, b.kept_cap);
   __shared__ int ovf;
   const int g = blockIdx.x;
   if (threadIdx.x == 0) ovf = 0;
@@ -488,11 +490,13 @@ void launch_binning(const BatchDev &b, int *tmp, double *F, hipStream_t s) {
 void launch_search(const BatchDev &b, const NbDev &nb, hipStream_t s, const int *active, const int *nactive,
                    int max_active) {
   const int n = active ? max_active : b.natoms;
-  if (n > 0) hipLaunchKernelGGL(k_search, dim3(n), dim3(64), 0, s, b, nb, active, nactive);
+  if (n > 0)
+    hipLaunchKernelGGL(k_search, dim3(n), dim3(64), 4 * (size_t)search_lds_ints(b.kept_cap), s, b, nb, active, nactive);
 }
 
 void launch_neighbor_lists(const BatchDev &b, const OutDev &o, hipStream_t s) {
-  if (b.natoms > 0) hipLaunchKernelGGL(k_neighbor_lists, dim3(b.natoms), dim3(64), 0, s, b, o);
+  if (b.natoms > 0)
+    hipLaunchKernelGGL(k_neighbor_lists, dim3(b.natoms), dim3(64), 4 * (size_t)search_lds_ints(b.kept_cap), s, b, o);
 }
 
 void launch_struct_energy(const BatchDev &b, const ModelDev &md, const double *E_atom, double *E_tot,

@@ -69,14 +69,29 @@ __device__ __forceinline__ void add_T_exact(const double *s, const int *T, doubl
   out[2] = s[2] + (double)T[2];
 }
 
-// LDS work space of one search
+// LDS work space of one search, sized by the batch's largest number of
+// (cell, image) combinations of a home cell (kc)
 struct SearchLds {
-  int kept[2 * AENET_AMD_KEPT_CAP];   // (cell, image rank) kept for this atom
-  int gk[AENET_AMD_KEPT_CAP];         // group: first kept entry
-  int gn[AENET_AMD_KEPT_CAP];         // group: number of images
-  int gpos[AENET_AMD_KEPT_CAP];       // group: first atom position in cell order
-  int gitems[AENET_AMD_KEPT_CAP + 1]; // group: prefix of (atoms x images)
+  int kc;
+  int *kept;    // [2*kc] (cell, image rank) kept for this atom
+  int *gk;      // [kc] group: first kept entry
+  int *gn;      // [kc] group: number of images
+  int *gpos;    // [kc] group: first atom position in cell order
+  int *gitems;  // [kc+1] group: prefix of (atoms x images)
 };
+
+__host__ __device__ inline int search_lds_ints(int kc) { return 6 * kc + 1; }
+
+__device__ inline SearchLds carve_search(int *p, int kc) {
+  SearchLds W;
+  W.kc = kc;
+  W.kept = p;
+  W.gk = p + 2 * kc;
+  W.gn = W.gk + kc;
+  W.gpos = W.gn + kc;
+  W.gitems = W.gpos + kc;
+  return W;
+}
 
 // Ordered neighbor search of atom g (global index) for one wavefront.  The
 // entries come out in the order lcl_nbdist_cart produces them
@@ -145,15 +160,15 @@ __device__ int search_neighbors(const BatchDev &b, int g, Sink &sink, SearchLds 
     }
     unsigned long long m = wave_ballot(keep);
     int pos = nk + mask_prefix(m);
-    if (keep && pos < AENET_AMD_KEPT_CAP) {
+    if (keep && pos < W.kc) {
       W.kept[2 * pos] = c;
       W.kept[2 * pos + 1] = t;
     }
     nk += popc64(m);
   }
-  if (nk > AENET_AMD_KEPT_CAP) {
+  if (nk > W.kc) {
     if (lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_KEPT_OVERFLOW);
-    nk = AENET_AMD_KEPT_CAP;
+    nk = W.kc;
   }
   __syncthreads();
   // groups = runs of equal cell; items = atoms x images of each group
