@@ -48,8 +48,10 @@ ModelDev DeviceModel::view() const {
 
 // fraction of a periodic cell's volume the atoms occupy: per lattice
 // direction one minus the widest empty gap between the atoms' fractional
-// coordinates (a slab's vacuum, wherever it sits in the cell)
-static double occupied_fraction(const double *L, int n, const double *cart) {
+// coordinates (a slab's vacuum, wherever it sits in the cell), counted
+// only when the gap is wider than the cutoff (the spacing of atomic
+// planes is not vacuum)
+static double occupied_fraction(const double *L, int n, const double *cart, double rc) {
   // L: column-major lattice (column k = vector k); fractional = L^-1 x
   const double det = L[0] * (L[4] * L[8] - L[7] * L[5]) - L[3] * (L[1] * L[8] - L[7] * L[2]) +
                      L[6] * (L[1] * L[5] - L[4] * L[2]);
@@ -74,7 +76,11 @@ static double occupied_fraction(const double *L, int n, const double *cart) {
     std::sort(f.begin(), f.end());
     double gap = 1.0 - (f[n - 1] - f[0]);
     for (int i = 1; i < n; ++i) gap = std::max(gap, f[i] - f[i - 1]);
-    occ *= std::max(0.05, 1.0 - gap);
+    // width of the cell along the normal of the other two vectors: |det| / |a_i x a_j|
+    const double *a = L + 3 * ((k + 1) % 3), *b = L + 3 * ((k + 2) % 3);
+    const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+    const double width = std::fabs(det) / std::sqrt(cx * cx + cy * cy + cz * cz);
+    if (gap * width > rc) occ *= std::max(0.05, 1.0 - gap);
   }
   return occ;
 }
@@ -86,7 +92,7 @@ int Batch::create(const PotSet &ps, int nstruct, const int *natoms, const double
   if (st) return st;
   occ_.assign(nstruct, 1.0);
   for (int s = 0, a0 = 0; s < nstruct; a0 += natoms[s], ++s)
-    if (pbc[s] && coords) occ_[s] = occupied_fraction(plan_.st[s].L, natoms[s], coords + 3 * (size_t)a0);
+    if (pbc[s] && coords) occ_[s] = occupied_fraction(plan_.st[s].L, natoms[s], coords + 3 * (size_t)a0, ps.Rc_max);
   return allocate(err);
 }
 
