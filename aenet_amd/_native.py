@@ -75,7 +75,7 @@ SIGNATURES = {
     "aenet_amd_model_check": (None, [_vp, _vp, _pint]),
     "aenet_amd_mc_create": (None, [_vp, _vp, C.c_int, C.c_int, _dp, _ip, _dp, C.c_ulonglong, C.POINTER(_vp), _pint]),
     "aenet_amd_mc_destroy": (None, [_vp]),
-    "aenet_amd_mc_set": (None, [_vp, C.c_double, C.c_double, C.c_double, _pint]),
+    "aenet_amd_mc_set": (None, [_vp, C.c_double, C.c_double, C.c_double, C.c_int, _pint]),
     "aenet_amd_mc_refresh": (None, [_vp, _pint]),
     "aenet_amd_mc_run": (None, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _pint]),
     "aenet_amd_mc_stats": (None, [_vp, C.POINTER(C.c_longlong), C.c_int, _pint]),

@@ -203,6 +203,7 @@ struct MCDev {
   int *types;       // [nchains*natoms]
   unsigned long long *rng;  // [nchains] generator states
   double beta, p_swap, max_disp, rc_t;
+  int max_screen;   // first-stage tries per chain and step (delayed acceptance)
   // the step's move per chain
   int *kind, *site_a, *site_b;
   double *old_pos;  // [3*nchains]
@@ -214,14 +215,16 @@ struct MCDev {
   // full potential: atomic energies (current, trial), chain energies
   double *E_t, *E_trial, *E_chain;
   // first-stage potential: unscaled descriptors and atomic energies
-  int nsf_s;
+  int nsf_s, rn_s, nh_s;  // features per atom; largest radial term count and hidden width
+  const void *fs_tab;     // first-stage table (mc.hip), built per run
   double *G_s, *G_trial, *E_s, *E_s_trial;
   int *affected;
   long long *stats;  // [4*nchains]: proposed, accepted, full evaluations, swaps
   long long *step;
-  double *trace;     // optional [max_trace*nchains*4]: dE_first, dE_full, passed stage 1, accepted
+  long long *nprop;  // [nchains] proposals resolved since the start of the run
+  double *trace;     // optional [max_trace*nchains*4] per proposal: dE_first, dE_full, stage-1 outcome, accepted
   long long max_trace;
-  double *samples;   // [max_samples*nchains] chain energies every sample_every steps
+  double *samples;   // [max_samples*nchains] chain energies every sample_every proposals
   int sample_every;
   long long max_samples;
 };

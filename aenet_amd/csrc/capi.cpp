@@ -1046,13 +1046,13 @@ void aenet_amd_mc_create(const aenet_amd_model *full, const aenet_amd_model *fir
     const PotSet &s = first->set;
     bool ok = s.desc == DESC_CHEBYSHEV && s.ntypes == full->set.ntypes && s.nsf_max <= 32;
     for (const SpeciesDev &sp : s.species) {
-      ok = ok && sp.a_Rc == 0.0f && sp.nlayers >= 2 && sp.nnodes[sp.nlayers - 1] == 1;
-      for (int l = 0; l < sp.nlayers; ++l) ok = ok && sp.nnodes[l] <= 32;
+      ok = ok && sp.a_Rc == 0.0f && sp.nlayers == 3 && sp.nnodes[1] <= 16 && sp.nnodes[2] == 1 && sp.r_N <= 16;
     }
     if (!ok) {
       *stat = set_err(AENET_AMD_ERR_ARG_VALUE,
                       "first-stage potential must be radial-only Chebyshev (angular Rc 0) with at most 32 "
-                      "features and layers of at most 32 nodes, for the same species as the full potential");
+                      "features and one hidden layer of at most 16 nodes, for the same species as the full "
+                      "potential");
       return;
     }
   }
@@ -1070,12 +1070,13 @@ void aenet_amd_mc_create(const aenet_amd_model *full, const aenet_amd_model *fir
 
 void aenet_amd_mc_destroy(aenet_amd_mc *mc) { delete mc; }
 
-void aenet_amd_mc_set(aenet_amd_mc *mc, double beta, double p_swap, double max_disp, int *stat) {
-  if (!mc || !(beta >= 0.0) || !(p_swap >= 0.0 && p_swap <= 1.0) || !(max_disp >= 0.0)) {
+void aenet_amd_mc_set(aenet_amd_mc *mc, double beta, double p_swap, double max_disp, int max_screen, int *stat) {
+  if (!mc || !(beta >= 0.0) || !(p_swap >= 0.0 && p_swap <= 1.0) || !(max_disp >= 0.0) || max_screen < 1 ||
+      max_screen > 1024) {
     *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_amd_mc_set: bad arguments");
     return;
   }
-  mc->mc.set_params(beta, p_swap, max_disp);
+  mc->mc.set_params(beta, p_swap, max_disp, max_screen);
   *stat = AENET_STATUS_OK;
 }
 
@@ -1123,19 +1124,19 @@ void aenet_amd_mc_samples(aenet_amd_mc *mc, double out[], long long *nsamples, i
   *stat = n < 0 ? set_err(AENET_STATUS_ERR_DEVICE, "device error") : AENET_STATUS_OK;
 }
 
-void aenet_amd_mc_trace(aenet_amd_mc *mc, long long max_steps, double out[], long long *nsteps, int *stat) {
+void aenet_amd_mc_trace(aenet_amd_mc *mc, long long max_props, double out[], long long *nprops, int *stat) {
   if (!mc) {
     *stat = set_err(AENET_STATUS_ERR_INIT, "no Monte Carlo state");
     return;
   }
-  if (max_steps >= 0) {
-    *stat = mc->mc.set_trace(max_steps);
+  if (max_props >= 0) {
+    *stat = mc->mc.set_trace(max_props);
     if (*stat) set_err(*stat, "device allocation failed");
-    if (nsteps) *nsteps = 0;
+    if (nprops) *nprops = 0;
     return;
   }
   long long n = mc->mc.trace(out);
-  if (nsteps) *nsteps = n < 0 ? 0 : n;
+  if (nprops) *nprops = n < 0 ? 0 : n;
   *stat = n < 0 ? set_err(AENET_STATUS_ERR_DEVICE, "device error") : AENET_STATUS_OK;
 }
 

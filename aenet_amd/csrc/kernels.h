@@ -51,6 +51,11 @@ void launch_adam(float *W, const float *g, float *m, float *v, int n, int t, flo
 
 // mc.hip: phase 0 = propose (+ first stage) + active lists, phase 1 = finish
 void launch_mc_step(const MCDev &m, const ModelDev *student, bool plain, int phase, hipStream_t s);
+void launch_mc_trace_init(double *trace, long long n, hipStream_t s);
+// the first-stage screening kernel's LDS holds a chain of natoms atoms
+bool mc_screen_fits(int natoms, int ntypes, int rn);
+size_t first_stage_table_bytes(int ntypes);
+void launch_first_stage_table(const ModelDev &student, void *tab, hipStream_t s);
 void launch_mc_student_init(const MCDev &m, const ModelDev &student, hipStream_t s);
 // neighbor lists (list radius rc_list) of the active atoms by minimum image
 void launch_mc_neighbors(const MCDev &m, const NbDev &nb, double rc_list, int *flags, hipStream_t s);

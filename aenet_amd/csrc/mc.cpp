@@ -89,6 +89,12 @@ int MonteCarlo::create(const PotSet &tset, const DeviceModel *teacher, const Dev
   if (student) {
     rc_s = student->rc_max;
     nsf_s_ = student->nsf_max;
+    rn_s_ = student->max_rN;
+    nh_s_ = student->hmax;
+    if (!mc_screen_fits(N, student->ntypes, rn_s_)) {
+      err = "cell too large for the first-stage kernel's LDS";
+      return AENET_STATUS_ERR_ARG;
+    }
   }
   // the first stage and the active-atom test use minimum images
   if (mc_min_width(L_) <= 2.0 * std::max(rc_t_, rc_s)) {
@@ -118,7 +124,8 @@ int MonteCarlo::create(const PotSet &tset, const DeviceModel *teacher, const Dev
   bad |= G_s_.alloc(n * nsf_s_) || G_trial_.alloc(n * nsf_s_) || E_s_.alloc(n) || E_s_trial_.alloc(n);
   bad |= kind_.alloc(C) || a_.alloc(C) || b_.alloc(C) || need_.alloc(C) || nactive_.alloc(1);
   bad |= active_.alloc(n) || act_start_.alloc(C) || act_count_.alloc(C) || affected_.alloc(n);
-  bad |= stats_.alloc(4 * (size_t)C) || step_.alloc(1);
+  bad |= stats_.alloc(4 * (size_t)C) || step_.alloc(1) || nprop_.alloc(C);
+  if (student) bad |= fs_tab_.alloc(first_stage_table_bytes(student->ntypes) / 8) != 0;
   std::vector<unsigned long long> r(C);
   for (int c = 0; c < C; ++c) r[c] = seed ^ (0x9E3779B97F4A7C15ull * (unsigned long long)(c + 1));
   bad |= rng_.upload(r.data(), C, stream_) != 0;
@@ -131,6 +138,7 @@ int MonteCarlo::create(const PotSet &tset, const DeviceModel *teacher, const Dev
   launch_fill_i32(act_count_.p, 0, C, stream_);
   launch_fill_i32(reinterpret_cast<int *>(stats_.p), 0, 8L * C, stream_);
   launch_fill_i32(reinterpret_cast<int *>(step_.p), 0, 2, stream_);
+  launch_fill_i32(reinterpret_cast<int *>(nprop_.p), 0, 2L * C, stream_);
   if (hipStreamSynchronize(stream_) != hipSuccess) {
     err = "device error";
     return AENET_STATUS_ERR_DEVICE;
@@ -138,10 +146,11 @@ int MonteCarlo::create(const PotSet &tset, const DeviceModel *teacher, const Dev
   return refresh(err);
 }
 
-void MonteCarlo::set_params(double beta, double p_swap, double max_disp) {
+void MonteCarlo::set_params(double beta, double p_swap, double max_disp, int max_screen) {
   beta_ = beta;
   p_swap_ = p_swap;
   max_disp_ = max_disp;
+  max_screen_ = max_screen;
   drop_graphs();
 }
 
@@ -158,6 +167,7 @@ MCDev MonteCarlo::view() const {
   m.beta = beta_;
   m.p_swap = p_swap_;
   m.max_disp = max_disp_;
+  m.max_screen = max_screen_;
   m.rc_t = rc_t_;
   m.kind = kind_.p;
   m.site_a = a_.p;
@@ -175,6 +185,9 @@ MCDev MonteCarlo::view() const {
   m.E_trial = E_trial_.p;
   m.E_chain = E_chain_.p;
   m.nsf_s = nsf_s_;
+  m.rn_s = rn_s_;
+  m.nh_s = nh_s_;
+  m.fs_tab = fs_tab_.p;
   m.G_s = G_s_.p;
   m.G_trial = G_trial_.p;
   m.E_s = E_s_.p;
@@ -182,6 +195,7 @@ MCDev MonteCarlo::view() const {
   m.affected = affected_.p;
   m.stats = stats_.p;
   m.step = step_.p;
+  m.nprop = nprop_.p;
   m.samples = sample_every_ > 0 ? samples_.p : nullptr;
   m.trace = max_trace_ > 0 ? trace_.p : nullptr;
   m.max_trace = max_trace_;
@@ -200,8 +214,10 @@ int MonteCarlo::refresh(std::string &err) {
   launch_mc_chain_sum(m, stream_);
   st = batch_.check_flags(stream_, err);
   if (st) return st;
-  if (student_) launch_mc_student_init(m, student_->view(), stream_);
-  first_fresh_ = student_ != nullptr;
+  if (student_) {
+    launch_first_stage_table(student_->view(), fs_tab_.p, stream_);
+    launch_mc_student_init(m, student_->view(), stream_);
+  }
   if (hipStreamSynchronize(stream_) != hipSuccess) {
     err = "device error";
     return AENET_STATUS_ERR_DEVICE;
@@ -230,17 +246,23 @@ int MonteCarlo::run(int nsteps, bool delayed, int sample_every, bool graph, std:
   // samples of this run; the step counter restarts
   drop_graphs();
   sample_every_ = sample_every > 0 ? sample_every : 0;
-  max_samples_ = sample_every_ ? nsteps / sample_every_ : 0;
+  const long long max_props = (long long)nsteps * (delayed ? max_screen_ : 1);
+  max_samples_ = sample_every_ ? max_props / sample_every_ : 0;
   if (max_samples_ > 0 && samples_.alloc((size_t)max_samples_ * C_)) {
     err = "device allocation failed";
     return AENET_STATUS_ERR_MALLOC;
   }
   launch_fill_i32(reinterpret_cast<int *>(step_.p), 0, 2, stream_);
+  launch_fill_i32(reinterpret_cast<int *>(nprop_.p), 0, 2L * C_, stream_);
+  if (max_trace_ > 0) launch_mc_trace_init(trace_.p, max_trace_ * C_ * 4, stream_);
   const MCDev m = view();
   const int k = delayed ? 1 : 0;
-  // plain steps move atoms without updating the first stage
-  if (delayed && !first_fresh_) launch_mc_student_init(m, student_->view(), stream_);
-  first_fresh_ = delayed;
+  // the first stage from scratch: its weights may have changed since the
+  // last run, and plain steps move atoms without updating its state
+  if (delayed) {
+    launch_first_stage_table(student_->view(), fs_tab_.p, stream_);
+    launch_mc_student_init(m, student_->view(), stream_);
+  }
   int st = AENET_STATUS_OK;
   if (graph) {
     if (hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal) != hipSuccess) {
@@ -273,12 +295,12 @@ int MonteCarlo::run(int nsteps, bool delayed, int sample_every, bool graph, std:
   return batch_.check_flags(stream_, err);
 }
 
-int MonteCarlo::set_trace(long long max_steps) {
+int MonteCarlo::set_trace(long long max_props) {
   drop_graphs();
   max_trace_ = 0;
-  if (max_steps > 0) {
-    if (trace_.alloc((size_t)max_steps * C_ * 4)) return AENET_STATUS_ERR_MALLOC;
-    max_trace_ = max_steps;
+  if (max_props > 0) {
+    if (trace_.alloc((size_t)max_props * C_ * 4)) return AENET_STATUS_ERR_MALLOC;
+    max_trace_ = max_props;
   }
   return AENET_STATUS_OK;
 }
@@ -317,11 +339,17 @@ int MonteCarlo::energies(double *E_chain) {
 
 long long MonteCarlo::samples(double *out) {
   if (max_samples_ <= 0) return 0;
-  if (out && (hipMemcpyAsync(out, samples_.p, (size_t)max_samples_ * C_ * sizeof(double), hipMemcpyDeviceToHost,
-                             stream_) != hipSuccess ||
-              hipStreamSynchronize(stream_) != hipSuccess))
+  std::vector<long long> np(C_);
+  if (hipMemcpyAsync(np.data(), nprop_.p, C_ * sizeof(long long), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+      hipStreamSynchronize(stream_) != hipSuccess)
     return -1;
-  return max_samples_;
+  long long n = max_samples_;
+  for (int c = 0; c < C_; ++c) n = std::min(n, np[c] / sample_every_);
+  if (out && n > 0 &&
+      (hipMemcpyAsync(out, samples_.p, (size_t)n * C_ * sizeof(double), hipMemcpyDeviceToHost, stream_) != hipSuccess ||
+       hipStreamSynchronize(stream_) != hipSuccess))
+    return -1;
+  return n;
 }
 
 int MonteCarlo::state(double *coords, int *types) {

@@ -23,18 +23,22 @@ class MonteCarlo {
   int create(const PotSet &teacher_set, const DeviceModel *teacher, const DeviceModel *student, int C, int N,
              const double *latt, const int *types, const double *coords, unsigned long long seed,
              std::string &err);
-  void set_params(double beta, double p_swap, double max_disp);
+  void set_params(double beta, double p_swap, double max_disp, int max_screen);
   // energies (and first-stage descriptors) from scratch
   int refresh(std::string &err);
-  // nsteps moves of every chain; samples of the chain energies every
-  // sample_every steps (0: none); graph: capture one step, replay it
+  // nsteps steps; a plain step makes one proposal per chain, a
+  // delayed-acceptance step up to max_screen (the first-stage tries up to
+  // the first that passes); samples of the chain energies every
+  // sample_every proposals of each chain (0: none); graph: capture one
+  // step, replay it
   int run(int nsteps, bool delayed, int sample_every, bool graph, std::string &err);
   int stats(long long out[4]);
   void reset_stats();
   int energies(double *E_chain);
-  long long samples(double *out);  // returns the count of samples per chain
-  // per-step, per-chain record of the decisions of the next run (0: off)
-  int set_trace(long long max_steps);
+  // the samples every chain completed (returns their count per chain)
+  long long samples(double *out);
+  // per-proposal, per-chain record of the decisions of the next runs (0: off)
+  int set_trace(long long max_props);
   long long trace(double *out);
   int state(double *coords, int *types);
   int student_state(double *E_s, double *G_s);
@@ -49,16 +53,16 @@ class MonteCarlo {
   Batch batch_;
   hipStream_t stream_ = nullptr;
   DBuf<double> coords_, old_pos_, u1_, u2_, dEs_, E_t_, E_trial_, E_chain_, G_s_, G_trial_, E_s_, E_s_trial_;
-  DBuf<double> samples_, E_struct_, trace_;
+  DBuf<double> samples_, E_struct_, trace_, fs_tab_;  // fs_tab_: first-stage table (bytes / 8)
   long long max_trace_ = 0;
   DBuf<int> types_, kind_, a_, b_, need_, nactive_, active_, act_start_, act_count_, affected_;
   DBuf<unsigned long long> rng_;
-  DBuf<long long> stats_, step_;
-  int C_ = 0, N_ = 0, nsf_s_ = 1;
+  DBuf<long long> stats_, step_, nprop_;
+  int C_ = 0, N_ = 0, nsf_s_ = 1, rn_s_ = 0, nh_s_ = 0;
   double L_[9] = {0}, Binv_[9] = {0};
   double rc_t_ = 0.0;
-  bool first_fresh_ = false;  // first-stage state matches the chains
   double beta_ = 1.0 / (8.617333262e-5 * 600.0), p_swap_ = 0.5, max_disp_ = 0.075;
+  int max_screen_ = 4;
   int sample_every_ = 0;
   long long max_samples_ = 0;
   hipGraphExec_t exec_[2] = {nullptr, nullptr};

@@ -2,25 +2,37 @@
 // the GPU, plain and delayed-acceptance (two-stage) with a cheap radial-only
 // potential as the first stage.
 //
-// One MC step moves every chain once:
-//   k_mc_propose : per chain, a swap of two atoms of different species or a
-//                  displacement of one atom by U(-d, d)^3, applied in place
-//                  (the old values are kept for a revert); SplitMix64
-//                  generator per chain, state in device memory
-//   k_mc_student : (delayed acceptance) per chain, one thread per atom: the
-//                  radial Chebyshev descriptor of every atom is kept, and
-//                  only the pair terms with the moved sites change it, so
-//                  the first-stage energy change needs no neighbor search;
-//                  the moved atoms are recomputed over the whole cell;
-//                  per-thread network; stage-1 decision exp(-beta dE_s)
+// One MC step sends at most one proposal per chain to the full potential:
+//   k_mc_propose : (plain) per chain, a swap of two atoms of different
+//                  species or a displacement of one atom by U(-d, d)^3,
+//                  applied in place (the old values are kept for a
+//                  revert); SplitMix64 generator per chain, state in device
+//                  memory
+//   k_mc_screen  : (delayed acceptance) per chain, one thread per atom:
+//                  proposals drawn from the chain's current state are
+//                  screened with the first-stage energy change, exp(-beta
+//                  dE_s), until one passes (at most max_screen per step);
+//                  one rejected here leaves the chain where it is and is a
+//                  complete MC step of that chain, so the chain's sequence
+//                  of proposals and decisions is that of sequential delayed
+//                  acceptance, while the full potential gets a proposal from
+//                  nearly every chain in every step.  The radial Chebyshev
+//                  descriptor of every atom is kept, and only the pair terms
+//                  with the moved sites change it, so the first-stage change
+//                  needs no neighbor search; the moved atoms are recomputed
+//                  over the whole cell; per-thread network
 //   k_mc_active  : per chain that goes to the full potential, the atoms
 //                  within its cutoff of a moved site, as one contiguous
 //                  block of a batch-wide list
 //   (full potential on the listed atoms: the batch's active-atom path)
 //   k_mc_finish  : per chain, dE of the full potential over its block; the
 //                  decision (plain: exp(-beta dE); second stage:
-//                  exp(-beta (dE - dE_s))); commit or revert; statistics and
-//                  energy samples
+//                  exp(-beta (dE - dE_s))); commit or revert
+// Statistics, the decision record and the energy samples are kept per
+// resolved proposal, in each chain's own count of proposals since the start
+// of the run (the chains of a delayed-acceptance run advance by different
+// numbers of proposals per step; sampling at step boundaries would weight
+// states by their chance of passing the first stage).
 // The second stage makes the chain exact for the full potential whatever
 // the first-stage potential is (Christen & Fox, J. Comput. Graph. Stat.
 // 14 (2005) 795).
@@ -31,6 +43,7 @@
 #include <algorithm>
 
 #include "aenet_amd_internal.h"
+#include "atom_common.h"
 #include "kernels.h"
 #include "kernels_common.h"
 
@@ -86,10 +99,10 @@ __device__ __forceinline__ void sdesc_zero(SDesc &G) {
 }
 
 // radial Chebyshev terms of one pair (sfb_eval radial part,
-// src/ext/sfbasis.f90:489-522): r_n += w_plain fc T_n, s_n += w_spin fc T_n
-__device__ __forceinline__ void student_pair(const SpeciesDev &sp, double r, double w_plain, double w_spin,
-                                             SDesc &G) {
-  const double Rc = sp.r_Rc;
+// src/ext/sfbasis.f90:489-522): vr_n += w_plain fc T_n, vs_n += w_spin fc T_n
+// for n < rN, cutoff Rc
+__device__ __forceinline__ void radial_terms(double Rc, int rN, double r, double w_plain, double w_spin, double *vr,
+                                             double *vs) {
   if (!(r <= Rc && r > 1.0e-12)) return;
   const double fc = 0.5 * (cos(3.14159265358979 * r / Rc) + 1.0);
   const double x = 2.0 * r / Rc - 1.0;
@@ -107,11 +120,47 @@ __device__ __forceinline__ void student_pair(const SpeciesDev &sp, double r, dou
       T = Tn;
       tn = Tn;
     }
-    if (n < sp.r_N) {
-      G.r[n] += w_plain * fc * tn;
-      G.s[n] += w_spin * fc * tn;
+    if (n < rN) {
+      vr[n] += w_plain * fc * tn;
+      vs[n] += w_spin * fc * tn;
     }
   }
+}
+
+// the same terms in FP32 arithmetic for the Monte Carlo first stage (a
+// screen only: the second stage keeps the chain exact), summed into the
+// FP64 descriptors vr, vs (plain and typespin)
+template <int RN, class T>
+__device__ __forceinline__ void radial_terms_f(float Rc, int rN, double r, float w_plain, float w_spin, T *vr, T *vs) {
+  const float rf = (float)r;
+  if (!(rf <= Rc && rf > 1.0e-12f)) return;
+  const float fc = 0.5f * (__cosf(3.14159265f * rf / Rc) + 1.0f);
+  const float x = 2.0f * rf / Rc - 1.0f;
+  float Tm = 1.0f, Tc = x;
+#pragma unroll
+  for (int n = 0; n < RN; ++n) {
+    float tn;
+    if (n == 0) {
+      tn = 1.0f;
+    } else if (n == 1) {
+      tn = x;
+    } else {
+      const float Tn = 2.0f * x * Tc - Tm;
+      Tm = Tc;
+      Tc = Tn;
+      tn = Tn;
+    }
+    if (n < rN) {
+      const float t = fc * tn;
+      vr[n] += (T)(w_plain * t);
+      vs[n] += (T)(w_spin * t);
+    }
+  }
+}
+
+__device__ __forceinline__ void student_pair(const SpeciesDev &sp, double r, double w_plain, double w_spin,
+                                             SDesc &G) {
+  radial_terms(sp.r_Rc, sp.r_N, r, w_plain, w_spin, G.r, G.s);
 }
 
 __device__ __forceinline__ float act_f(int f, float z) {
@@ -125,65 +174,148 @@ __device__ __forceinline__ float act_f(int f, float z) {
   }
 }
 
+// first-stage potential of one species in compact form, held in LDS by the
+// Monte Carlo kernels: the radial basis, the normalization of its plain
+// and typespin features, the one-hidden-layer network (weights k-major;
+// the constant input of the zero angular features folded into the hidden
+// bias) and the energy epilogue
+struct FirstStage {
+  double avg[2 * SR], scl[2 * SR];  // plain features n, typespin features SR + n
+  double E_scale, E_shift, E_atom;
+  float W[SW][2 * SR];              // hidden node k: weights of the features
+  float b[SW], Wo[SW];
+  float bo, Rc;
+  float spin[AENET_AMD_MAX_TYPES];
+  int rN, multi, nh, fa_h, fa_o, r_i1, r_i2, pad;
+};
+static_assert(sizeof(FirstStage) % 8 == 0, "FirstStage keeps the doubles after it aligned");
+typedef __attribute__((address_space(4))) const FirstStage ConstFirstStage;
+
+// the workgroup fills tab[0 .. ntypes-1] from the model (one hidden layer)
+__device__ void load_first_stage(const ModelDev &md, FirstStage *tab) {
+  for (int t = 0; t < md.ntypes; ++t) {
+    const SpeciesDev &sp = md.species[t];
+    FirstStage &S = tab[t];
+    const int nh = sp.nnodes[1];
+    const float *W0 = md.W + sp.w_off[0];
+    const float *W1 = md.W + sp.w_off[1];
+    const double *avg = md.norm_avg + sp.norm_off;
+    const double *scl = md.norm_scale + sp.norm_off;
+    for (int i = threadIdx.x; i < SW * 2 * SR; i += blockDim.x) {
+      const int k = i / (2 * SR), f = i - k * 2 * SR;
+      const int q = f >= SR, n = f - q * SR;
+      float w = 0.0f;
+      if (n < sp.r_N && k < nh && (q == 0 || sp.multi)) w = W0[((q ? sp.r_i2 : sp.r_i1) + n) * nh + k];
+      S.W[k][f] = w;
+    }
+    for (int f = threadIdx.x; f < 2 * SR; f += blockDim.x) {
+      const int q = f >= SR, n = f - q * SR;
+      const bool on = n < sp.r_N && (q == 0 || sp.multi);
+      const int i = (q ? sp.r_i2 : sp.r_i1) + n;
+      S.avg[f] = on ? avg[i] : 0.0;
+      S.scl[f] = on ? scl[i] : 0.0;
+    }
+    if (threadIdx.x < SW) {
+      const int k = threadIdx.x;
+      float b = 0.0f, wo = 0.0f;
+      if (k < nh) {
+        b = W0[sp.nnodes[0] * nh + k];
+        for (int n = 0; n < sp.a_N; ++n) {
+          b += W0[(sp.a_i1 + n) * nh + k] * (float)(scl[sp.a_i1 + n] * (0.0 - avg[sp.a_i1 + n]));
+          if (sp.multi) b += W0[(sp.a_i2 + n) * nh + k] * (float)(scl[sp.a_i2 + n] * (0.0 - avg[sp.a_i2 + n]));
+        }
+        wo = W1[k];
+      }
+      S.b[k] = b;
+      S.Wo[k] = wo;
+    }
+    if (threadIdx.x == 0) {
+      S.E_scale = sp.E_scale;
+      S.E_shift = sp.E_shift;
+      S.E_atom = sp.E_atom;
+      S.bo = W1[nh];
+      S.Rc = sp.r_Rc;
+      for (int u = 0; u < AENET_AMD_MAX_TYPES; ++u) S.spin[u] = sp.spin[u];
+      S.rN = sp.r_N;
+      S.multi = sp.multi;
+      S.nh = nh;
+      S.fa_h = sp.fa[1];
+      S.fa_o = sp.fa[2];
+      S.r_i1 = sp.r_i1;
+      S.r_i2 = sp.r_i2;
+      S.pad = 0;
+    }
+  }
+}
+
+// the workgroup copies the table into LDS
+__device__ __forceinline__ void copy_first_stage(const FirstStage *src, int ntypes, FirstStage *dst) {
+  const double *a = reinterpret_cast<const double *>(src);
+  double *b = reinterpret_cast<double *>(dst);
+  const int n = ntypes * (int)(sizeof(FirstStage) / 8);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) b[k] = a[k];
+}
+
 // the first stage's atomic energy from its unscaled descriptor:
 // normalization (stp_normalize), ff_eval in per-thread FP32 registers,
-// energy epilogue in FP64
-__device__ __forceinline__ double student_energy(const ModelDev &md, const SpeciesDev &sp, const SDesc &G) {
-  const double *avg = md.norm_avg + sp.norm_off;
-  const double *scl = md.norm_scale + sp.norm_off;
-  float xr[SR], xs[SR];
+// energy epilogue in FP64; RN radial terms and NH hidden nodes at most
+// (the table's entries beyond the species' own sizes are zero)
+template <int RN, int NH, class Table>
+__device__ __forceinline__ double fs_energy(const Table &S, const SDesc &G) {
+  float x[2 * RN];
+#pragma unroll
+  for (int n = 0; n < RN; ++n) {
+    x[n] = (float)(S.scl[n] * (G.r[n] - S.avg[n]));
+    x[RN + n] = (float)(S.scl[SR + n] * (G.s[n] - S.avg[SR + n]));
+  }
+  float y = S.bo;
+#pragma unroll
+  for (int k = 0; k < NH; ++k) {
+    float z = S.b[k];
+#pragma unroll
+    for (int n = 0; n < RN; ++n) z += S.W[k][n] * x[n];
+#pragma unroll
+    for (int n = 0; n < RN; ++n) z += S.W[k][SR + n] * x[RN + n];
+    y += S.Wo[k] * act_f(S.fa_h, z);
+  }
+  y = act_f(S.fa_o, y);
+  return S.E_scale * (double)y + S.E_shift + S.E_atom;
+}
+
+// the first-stage descriptors of a chain's atoms: a copy in LDS (stride
+// 2*rn: plain terms, then typespin terms) or the global array (new_SFBasis
+// layout, nsf per atom)
+struct FSRead {
+  const double *G;
+  int stride, rn;
+  bool lds;
+};
+
+template <int RN>
+__device__ __forceinline__ void fs_load(const FSRead &R, const FirstStage &S, int j, SDesc &G) {
+  const double *p = R.G + (size_t)j * R.stride;
+#pragma unroll
+  for (int n = 0; n < RN; ++n) {
+    if (R.lds) {
+      G.r[n] = n < R.rn ? p[n] : 0.0;
+      G.s[n] = n < R.rn ? p[R.rn + n] : 0.0;
+    } else {
+      G.r[n] = n < S.rN ? p[S.r_i1 + n] : 0.0;
+      G.s[n] = (n < S.rN && S.multi) ? p[S.r_i2 + n] : 0.0;
+    }
+  }
+}
+
+// stores the features in the new_SFBasis layout (angular ones zero)
+__device__ __forceinline__ void fs_store(const FirstStage &S, int nsf, double *dst, const SDesc &G) {
+  for (int f = 0; f < nsf; ++f) dst[f] = 0.0;
 #pragma unroll
   for (int n = 0; n < SR; ++n) {
-    xr[n] = n < sp.r_N ? (float)(scl[sp.r_i1 + n] * (G.r[n] - avg[sp.r_i1 + n])) : 0.0f;
-    xs[n] = (n < sp.r_N && sp.multi) ? (float)(scl[sp.r_i2 + n] * (G.s[n] - avg[sp.r_i2 + n])) : 0.0f;
-  }
-  float h[SW];
-  {
-    const int n1 = sp.nnodes[0], n2 = sp.nnodes[1];
-    const float *W = md.W + sp.w_off[0];
-#pragma unroll
-    for (int k = 0; k < SW; ++k) {
-      if (k < n2) {
-        float z = W[n1 * n2 + k];
-#pragma unroll
-        for (int n = 0; n < SR; ++n) {
-          if (n < sp.r_N) {
-            z += W[(sp.r_i1 + n) * n2 + k] * xr[n];
-            if (sp.multi) z += W[(sp.r_i2 + n) * n2 + k] * xs[n];
-          }
-        }
-        // angular features of a radial-only basis: G = 0
-        for (int n = 0; n < sp.a_N; ++n) {
-          z += W[(sp.a_i1 + n) * n2 + k] * (float)(scl[sp.a_i1 + n] * (0.0 - avg[sp.a_i1 + n]));
-          if (sp.multi) z += W[(sp.a_i2 + n) * n2 + k] * (float)(scl[sp.a_i2 + n] * (0.0 - avg[sp.a_i2 + n]));
-        }
-        h[k] = act_f(sp.fa[1], z);
-      } else {
-        h[k] = 0.0f;
-      }
+    if (n < S.rN) {
+      dst[S.r_i1 + n] = G.r[n];
+      if (S.multi) dst[S.r_i2 + n] = G.s[n];
     }
   }
-  for (int l = 1; l + 1 < sp.nlayers; ++l) {
-    const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
-    const float *W = md.W + sp.w_off[l];
-    float o[SW];
-#pragma unroll
-    for (int k = 0; k < SW; ++k) {
-      if (k < n2) {
-        float z = W[n1 * n2 + k];
-#pragma unroll
-        for (int c = 0; c < SW; ++c)
-          if (c < n1) z += W[c * n2 + k] * h[c];
-        o[k] = act_f(sp.fa[l + 1], z);
-      } else {
-        o[k] = 0.0f;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < SW; ++k) h[k] = o[k];
-  }
-  double E = sp.E_scale * (double)h[0] + sp.E_shift;
-  return E + sp.E_atom;
 }
 
 // descriptor of atom j over every other atom of its chain (positions in LDS)
@@ -196,24 +328,158 @@ __device__ __forceinline__ void student_full(const MCDev &m, const SpeciesDev &s
   }
 }
 
-__device__ __forceinline__ void sdesc_load(const MCDev &m, const SpeciesDev &sp, size_t g, SDesc &G) {
-  const double *p = m.G_s + g * m.nsf_s;
+// one proposal: kind, sites, displacement, and the uniforms of the two
+// acceptance tests
+struct Move {
+  int kind, a, b;
+  double d[3];
+  double u1, u2;
+};
+
+// a proposal from the chain's current types pt: a swap of two atoms of
+// different species with probability p_swap (kind MC_NONE when 64 draws
+// find no atom of another species), else a displacement of one atom by
+// U(-max_disp, max_disp)^3
+__device__ __forceinline__ Move draw_move(const MCDev &m, const int *pt, uint64_t &s) {
+  Move mv;
+  const int N = m.natoms;
+  mv.kind = MC_NONE;
+  mv.a = 0;
+  mv.b = 0;
+  mv.d[0] = mv.d[1] = mv.d[2] = 0.0;
+  if (uniform01(s) < m.p_swap) {
+    mv.a = min((int)(uniform01(s) * N), N - 1);
+    const int ta = pt[mv.a];
+    for (int k = 0; k < 64; ++k) {
+      const int cand = min((int)(uniform01(s) * N), N - 1);
+      if (pt[cand] != ta) {
+        mv.b = cand;
+        mv.kind = MC_SWAP;
+        break;
+      }
+    }
+  } else {
+    mv.kind = MC_DISP;
+    mv.a = min((int)(uniform01(s) * N), N - 1);
+    for (int x = 0; x < 3; ++x) mv.d[x] = (2.0 * uniform01(s) - 1.0) * m.max_disp;
+  }
+  mv.u1 = uniform01(s);
+  mv.u2 = uniform01(s);
+  return mv;
+}
+
+// the first-stage descriptor of atom j (species S) after the move mv:
+// positions sx and types pt of the chain with the move applied, xold the
+// displaced atom's old position, Gm the moved sites' descriptors, R the
+// chain's current descriptors; false (G untouched) when the move leaves
+// atom j's descriptor unchanged
+template <int RN>
+__device__ __forceinline__ bool trial_desc(const MCDev &m, const FirstStage &S, int j, const Move &mv,
+                                           const double *sx, const int *pt, const double *xold,
+                                           const double (*Gm)[2 * SR], const FSRead &R, SDesc &G) {
+  if (j == mv.a || (mv.kind == MC_SWAP && j == mv.b)) {
+    const int q = j == mv.a ? 0 : 1;
 #pragma unroll
-  for (int n = 0; n < SR; ++n) {
-    G.r[n] = n < sp.r_N ? p[sp.r_i1 + n] : 0.0;
-    G.s[n] = (n < sp.r_N && sp.multi) ? p[sp.r_i2 + n] : 0.0;
+    for (int n = 0; n < RN; ++n) {
+      G.r[n] = Gm[q][n];
+      G.s[n] = Gm[q][SR + n];
+    }
+    return true;
+  }
+  const double Rc = S.Rc;
+  if (mv.kind == MC_DISP) {
+    // atom a left xold for its new position
+    const double ro = dist(m, &sx[3 * j], xold);
+    const double rn = dist(m, &sx[3 * j], &sx[3 * mv.a]);
+    if (!(ro <= Rc || rn <= Rc)) return false;
+    fs_load<RN>(R, S, j, G);
+    const float spa = S.spin[pt[mv.a] - 1];
+    radial_terms_f<RN>(S.Rc, S.rN, ro, -1.0f, -spa, G.r, G.s);
+    radial_terms_f<RN>(S.Rc, S.rN, rn, 1.0f, spa, G.r, G.s);
+    return true;
+  }
+  // swap: a and b exchanged species; only the typespin weights of the
+  // pairs (j, a) and (j, b) change
+  const double ra = dist(m, &sx[3 * j], &sx[3 * mv.a]);
+  const double rb = dist(m, &sx[3 * j], &sx[3 * mv.b]);
+  if (!(ra <= Rc || rb <= Rc)) return false;
+  fs_load<RN>(R, S, j, G);
+  const float ds = S.spin[pt[mv.a] - 1] - S.spin[pt[mv.b] - 1];  // new minus old weight of the pair (j, a)
+  radial_terms_f<RN>(S.Rc, S.rN, ra, 0.0f, ds, G.r, G.s);
+  radial_terms_f<RN>(S.Rc, S.rN, rb, 0.0f, -ds, G.r, G.s);
+  return true;
+}
+
+// the moved sites' first-stage descriptors over the whole cell: per-lane
+// FP32 pair terms, a reduce-scatter over the wavefront (first level FP32,
+// then FP64; lane l ends with feature l/2), the waves' partials in
+// part[wave][q][f], summed in a fixed order into Gm by the caller
+template <int RN>
+__device__ __forceinline__ void moved_site_partials(const MCDev &m, const FirstStage *tab, const Move &mv,
+                                                    const double *sx, const int *pt, double *part) {
+  const int N = m.natoms;
+  const int lane = lane_id(), wave = threadIdx.x / WAVE;
+  for (int q = 0; q < 2; ++q) {
+    if (q == 1 && mv.kind != MC_SWAP) break;
+    const int st = q == 0 ? mv.a : mv.b;
+    const FirstStage &S = tab[pt[st] - 1];
+    float v[2 * SR];
+#pragma unroll
+    for (int f = 0; f < 2 * SR; ++f) v[f] = 0.0f;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+      if (j == st) continue;
+      const double r = dist(m, &sx[3 * st], &sx[3 * j]);
+      radial_terms_f<RN>(S.Rc, S.rN, r, 1.0f, S.spin[pt[j] - 1], v, v + SR);
+    }
+    const double t = reduce_scatter<2 * SR>(v);
+    if ((lane & 1) == 0) part[(wave * 2 + q) * 2 * SR + (lane >> 1)] = t;
   }
 }
 
-// stores the features in the new_SFBasis layout (angular ones zero)
-__device__ __forceinline__ void sdesc_store(const MCDev &m, const SpeciesDev &sp, double *dst, const SDesc &G) {
-  for (int f = 0; f < m.nsf_s; ++f) dst[f] = 0.0;
-#pragma unroll
-  for (int n = 0; n < SR; ++n) {
-    if (n < sp.r_N) {
-      dst[sp.r_i1 + n] = G.r[n];
-      if (sp.multi) dst[sp.r_i2 + n] = G.s[n];
-    }
+// per-chain bookkeeping held in registers by one thread while a kernel
+// resolves proposals: proposals since the start of the run, statistics
+// (proposed, accepted, full-potential evaluations, swaps) and the chain
+// energy after the latest decision
+struct ChainLog {
+  long long nprop, st[4];
+  double E;
+};
+
+__device__ __forceinline__ ChainLog log_open(const MCDev &m, int c) {
+  ChainLog L;
+  L.nprop = m.nprop[c];
+  for (int k = 0; k < 4; ++k) L.st[k] = 0;
+  L.E = m.E_chain[c];
+  return L;
+}
+
+__device__ __forceinline__ void log_close(const MCDev &m, int c, const ChainLog &L) {
+  m.nprop[c] = L.nprop;
+  for (int k = 0; k < 4; ++k)
+    if (L.st[k]) m.stats[4 * (size_t)c + k] += L.st[k];
+}
+
+// one resolved proposal of chain c: statistics, the decision record and
+// the energy samples, in the chain's own count of proposals since the
+// start of the run; L.E is the energy after the decision
+__device__ __forceinline__ void log_proposal(const MCDev &m, int c, ChainLog &L, int kind, double dE_first,
+                                             double dE_full, int stage1, bool accept) {
+  L.st[0] += 1;
+  if (accept) L.st[1] += 1;
+  if (stage1 == 1) L.st[2] += 1;
+  if (kind == MC_SWAP) L.st[3] += 1;
+  const long long k = L.nprop;
+  if (m.trace && k < m.max_trace) {
+    double *tr = m.trace + 4 * ((size_t)k * m.nchains + c);
+    tr[0] = dE_first;
+    tr[1] = dE_full;
+    tr[2] = (double)stage1;
+    tr[3] = accept ? 1.0 : 0.0;
+  }
+  L.nprop = k + 1;
+  if (m.samples && m.sample_every > 0 && (k + 1) % m.sample_every == 0) {
+    const long long q = (k + 1) / m.sample_every - 1;
+    if (q < m.max_samples) m.samples[(size_t)q * m.nchains + c] = L.E;
   }
 }
 
@@ -227,169 +493,216 @@ __global__ void k_mc_propose(MCDev m) {
   }
   if (c >= m.nchains) return;
   uint64_t s = m.rng[c];
-  const int N = m.natoms;
-  const size_t base = (size_t)c * N;
-  int kind = MC_NONE, a = 0, b = 0;
-  if (uniform01(s) < m.p_swap) {
-    a = min((int)(uniform01(s) * N), N - 1);
-    const int ta = m.types[base + a];
-    for (int k = 0; k < 64; ++k) {
-      const int cand = min((int)(uniform01(s) * N), N - 1);
-      if (m.types[base + cand] != ta) {
-        b = cand;
-        kind = MC_SWAP;
-        break;
-      }
-    }
-    if (kind == MC_SWAP) {
-      const int tb = m.types[base + b];
-      m.types[base + a] = tb;
-      m.types[base + b] = ta;
-    }
-  } else {
-    kind = MC_DISP;
-    a = min((int)(uniform01(s) * N), N - 1);
+  const size_t base = (size_t)c * m.natoms;
+  const Move mv = draw_move(m, m.types + base, s);
+  if (mv.kind == MC_SWAP) {
+    const int ta = m.types[base + mv.a];
+    m.types[base + mv.a] = m.types[base + mv.b];
+    m.types[base + mv.b] = ta;
+  } else if (mv.kind == MC_DISP) {
     for (int x = 0; x < 3; ++x) {
-      const double old = m.coords[3 * (base + a) + x];
+      const double old = m.coords[3 * (base + mv.a) + x];
       m.old_pos[3 * c + x] = old;
-      m.coords[3 * (base + a) + x] = old + (2.0 * uniform01(s) - 1.0) * m.max_disp;
+      m.coords[3 * (base + mv.a) + x] = old + mv.d[x];
     }
   }
-  m.kind[c] = kind;
-  m.site_a[c] = a;
-  m.site_b[c] = b;
-  m.u_stage1[c] = uniform01(s);
-  m.u_stage2[c] = uniform01(s);
+  m.kind[c] = mv.kind;
+  m.site_a[c] = mv.a;
+  m.site_b[c] = mv.b;
+  m.u_stage1[c] = mv.u1;
+  m.u_stage2[c] = mv.u2;
+  m.need[c] = mv.kind != MC_NONE ? 1 : 0;
   m.rng[c] = s;
 }
 
-// one workgroup per chain, threads over the chain's atoms
-__global__ __launch_bounds__(256) void k_mc_student(MCDev m, ModelDev md) {
-  extern __shared__ double sx[];  // [3*N] positions of the chain
-  __shared__ double red[16];
-  __shared__ double Gm[2][2 * SR];  // descriptors of the moved sites, summed over the workgroup
+// LDS of k_mc_screen in doubles: positions [3N], trial energies [N], wave
+// partials [4][2][2*SR], the first-stage tables, with the cache the current
+// energies [N] and descriptors [N][2*rn], then types [N] and affected
+// flags [N] (ints)
+__host__ __device__ inline size_t screen_lds_bytes(int N, int ntypes, int rn, bool cache) {
+  size_t d = 3 * (size_t)N + N + 4 * 2 * 2 * SR;
+  if (cache) d += (size_t)N * (1 + 2 * rn);
+  return 8 * d + ntypes * sizeof(FirstStage) + 2 * 4 * (size_t)N;
+}
+
+// the first-stage table of every species, built once per run (one
+// workgroup); the kernels read it with uniform addresses
+__global__ __launch_bounds__(256) void k_first_stage_table(ModelDev md, FirstStage *tab) { load_first_stage(md, tab); }
+
+// one workgroup per chain, threads over the chain's atoms; CACHE: the
+// chain's first-stage state is copied into LDS once per step; RN, NH: the
+// first stage's radial terms and hidden nodes at most.  The network runs
+// once per species present among a wavefront's affected atoms, with the
+// weights of tabg (uniform loads); the LDS copy of the table serves the
+// per-lane lookups
+template <bool CACHE, int RN, int NH>
+__global__ __launch_bounds__(256) void k_mc_screen(MCDev m, ModelDev md, const FirstStage *__restrict__ tabg) {
+  extern __shared__ double lds[];
+  __shared__ double Gm[2][2 * SR];  // descriptors of the moved sites
+  __shared__ double red[4];
+  __shared__ Move smv;
+  __shared__ double xold[3];
+  __shared__ int s_pass;
   const int c = blockIdx.x;
   const int N = m.natoms;
+  const int rn = m.rn_s;
   const size_t base = (size_t)c * N;
-  const int kind = m.kind[c];
+  const int nwaves = (blockDim.x + WAVE - 1) / WAVE;
+  double *sx = lds;
+  double *sE = sx + 3 * N;
+  double *part = sE + N;
+  FirstStage *tab = reinterpret_cast<FirstStage *>(part + 4 * 2 * 2 * SR);
+  double *Ec = reinterpret_cast<double *>(tab + md.ntypes);
+  double *Gc = Ec + (CACHE ? N : 0);
+  int *pt = reinterpret_cast<int *>(Gc + (CACHE ? (size_t)N * 2 * rn : 0));
+  int *saff = pt + N;
+  if (c == 0 && threadIdx.x == 0) {
+    *m.nactive = 0;
+    *m.step += 1;
+  }
+  copy_first_stage(tabg, md.ntypes, tab);
   for (int k = threadIdx.x; k < 3 * N; k += blockDim.x) sx[k] = m.coords[3 * base + k];
-  if (threadIdx.x < 4 * SR) (&Gm[0][0])[threadIdx.x] = 0.0;
+  for (int k = threadIdx.x; k < N; k += blockDim.x) pt[k] = m.types[base + k];
   __syncthreads();
-  // the moved sites' descriptors: every other atom adds its pair term
-  if (kind != MC_NONE) {
-    const int a = m.site_a[c], b = m.site_b[c];
-    const int ta = m.types[base + a];
-    const int tb = m.types[base + b];
-    for (int j = threadIdx.x; j < N; j += blockDim.x) {
-      const int tj = m.types[base + j];
-      if (j != a) {
-        const SpeciesDev &spa = md.species[ta - 1];
-        const double r = dist(m, &sx[3 * a], &sx[3 * j]);
-        if (r <= spa.r_Rc) {
-          SDesc t;
-          sdesc_zero(t);
-          student_pair(spa, r, 1.0, spa.spin[tj - 1], t);
-#pragma unroll
-          for (int n = 0; n < SR; ++n) {
-            if (n < spa.r_N) {
-              unsafeAtomicAdd(&Gm[0][n], t.r[n]);
-              unsafeAtomicAdd(&Gm[0][SR + n], t.s[n]);
-            }
-          }
-        }
-      }
-      if (kind == MC_SWAP && j != b) {
-        const SpeciesDev &spb = md.species[tb - 1];
-        const double r = dist(m, &sx[3 * b], &sx[3 * j]);
-        if (r <= spb.r_Rc) {
-          SDesc t;
-          sdesc_zero(t);
-          student_pair(spb, r, 1.0, spb.spin[tj - 1], t);
-#pragma unroll
-          for (int n = 0; n < SR; ++n) {
-            if (n < spb.r_N) {
-              unsafeAtomicAdd(&Gm[1][n], t.r[n]);
-              unsafeAtomicAdd(&Gm[1][SR + n], t.s[n]);
-            }
-          }
-        }
-      }
+  if (CACHE) {
+    for (int k = threadIdx.x; k < N * 2 * rn; k += blockDim.x) {
+      const int j = k / (2 * rn), f = k - j * 2 * rn;
+      const FirstStage &S = tab[pt[j] - 1];
+      const int q = f >= rn, n = f - q * rn;
+      Gc[k] = (n < S.rN && (q == 0 || S.multi)) ? m.G_s[(base + j) * m.nsf_s + (q ? S.r_i2 : S.r_i1) + n] : 0.0;
     }
+    for (int k = threadIdx.x; k < N; k += blockDim.x) Ec[k] = m.E_s[base + k];
   }
-  __syncthreads();
-  double dE = 0.0;
-  for (int j = threadIdx.x; j < N; j += blockDim.x) {
-    const size_t g = base + j;
-    if (kind == MC_NONE) {
-      m.affected[g] = 0;
-      continue;
-    }
-    const int a = m.site_a[c], b = m.site_b[c];
-    const int tj = m.types[g];  // new type (swaps already applied)
-    const SpeciesDev &sp = md.species[tj - 1];
-    SDesc G;
-    bool affected = false;
-    const bool full = (j == a) || (kind == MC_SWAP && j == b);
-    if (full) {
-      const int q = j == a ? 0 : 1;
-#pragma unroll
-      for (int n = 0; n < SR; ++n) {
-        G.r[n] = Gm[q][n];
-        G.s[n] = Gm[q][SR + n];
-      }
-      affected = true;
-    } else {
-      sdesc_load(m, sp, g, G);
-      if (kind == MC_DISP) {
-        // atom a left old_pos for its new position
-        const double ro = dist(m, &sx[3 * j], &m.old_pos[3 * c]);
-        const double rn = dist(m, &sx[3 * j], &sx[3 * a]);
-        if (ro <= sp.r_Rc || rn <= sp.r_Rc) {
-          const double spa = sp.spin[m.types[base + a] - 1];
-          student_pair(sp, ro, -1.0, -spa, G);
-          student_pair(sp, rn, 1.0, spa, G);
-          affected = true;
-        }
-      } else {
-        // swap: a and b exchanged species in place; only the typespin
-        // weights of the pairs (j, a) and (j, b) change
-        const int sites[2] = {a, b};
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const int st = sites[q];
-          const double r = dist(m, &sx[3 * j], &sx[3 * st]);
-          if (r <= sp.r_Rc) {
-            const double s_new = sp.spin[m.types[base + st] - 1];
-            const double s_old = sp.spin[m.types[base + sites[1 - q]] - 1];
-            student_pair(sp, r, 0.0, s_new - s_old, G);
-            affected = true;
-          }
-        }
-      }
-    }
-    if (affected) {
-      const double E = student_energy(md, sp, G);
-      dE += E - m.E_s[g];
-      sdesc_store(m, sp, m.G_trial + g * m.nsf_s, G);
-      m.E_s_trial[g] = E;
-    }
-    m.affected[g] = affected ? 1 : 0;
-  }
-  // chain sum of dE_s
-  double v = wave_sum(dE);
-  if (lane_id() == 0) red[threadIdx.x / WAVE] = v;
-  __syncthreads();
+  const FSRead R = CACHE ? FSRead{Gc, 2 * rn, rn, true} : FSRead{m.G_s + base * m.nsf_s, m.nsf_s, rn, false};
+  const double *Ecur = CACHE ? Ec : m.E_s + base;
+  uint64_t s = 0;
+  ChainLog L;  // thread 0
   if (threadIdx.x == 0) {
-    double t = 0.0;
-    for (int w = 0; w < (int)((blockDim.x + WAVE - 1) / WAVE); ++w) t += red[w];
-    m.dE_s[c] = t;
-    bool pass = kind != MC_NONE && log(m.u_stage1[c]) < -m.beta * t;
-    m.need[c] = pass ? 1 : 0;
+    s = m.rng[c];
+    L = log_open(m, c);
+  }
+  double dE_pass = 0.0;  // thread 0: first-stage change of the proposal that passed
+  bool pass = false;
+  for (int t = 0; t < m.max_screen && !pass; ++t) {
+    __syncthreads();  // the previous try's shared values are no longer read
+    if (threadIdx.x == 0) {
+      const Move mv = draw_move(m, pt, s);
+      if (mv.kind == MC_SWAP) {
+        const int ta = pt[mv.a];
+        pt[mv.a] = pt[mv.b];
+        pt[mv.b] = ta;
+      } else if (mv.kind == MC_DISP) {
+        for (int x = 0; x < 3; ++x) {
+          xold[x] = sx[3 * mv.a + x];
+          sx[3 * mv.a + x] = xold[x] + mv.d[x];
+        }
+      }
+      smv = mv;
+    }
+    __syncthreads();
+    const Move mv = smv;
+    if (mv.kind == MC_NONE) continue;  // no atom of another species: not a proposal
+    moved_site_partials<RN>(m, tab, mv, sx, pt, part);
+    __syncthreads();
+    if (threadIdx.x < 4 * SR) {
+      const int q = threadIdx.x / (2 * SR), f = threadIdx.x - q * 2 * SR;
+      double g = 0.0;
+      for (int w = 0; w < nwaves; ++w) g += part[(w * 2 + q) * 2 * SR + f];
+      Gm[q][f] = g;
+    }
+    __syncthreads();
+    double dE = 0.0;
+    // uniform loops, so that the species index and the table address are
+    // wave-uniform (scalar loads of the weights)
+    for (int j0 = 0; j0 < N; j0 += blockDim.x) {
+      const int j = j0 + threadIdx.x;
+      const bool on = j < N;
+      const int tj = on ? pt[j] - 1 : 0;
+      SDesc G;
+      const bool aff = on && trial_desc<RN>(m, tab[tj], j, mv, sx, pt, xold, Gm, R, G);
+      double E = 0.0;
+      for (int t = 0; t < md.ntypes; ++t) {
+        const bool mine = aff && tj == t;
+        if (wave_ballot(mine)) {
+          // the species' table through the constant address space: scalar loads
+          const ConstFirstStage *St = (const ConstFirstStage *)(tabg + uniform(t));
+          if (mine) E = fs_energy<RN, NH>(*St, G);
+        }
+      }
+      if (aff) {
+        sE[j] = E;
+        dE += E - Ecur[j];
+      }
+      if (on) saff[j] = aff ? 1 : 0;
+    }
+    dE = wave_sum(dE);
+    if (lane_id() == 0) red[threadIdx.x / WAVE] = dE;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0;
+      for (int w = 0; w < nwaves; ++w) tot += red[w];
+      const bool p = log(mv.u1) < -m.beta * tot;
+      s_pass = p ? 1 : 0;
+      dE_pass = tot;
+      if (!p) {
+        // rejected by the first stage: the chain stays where it is
+        log_proposal(m, c, L, mv.kind, tot, 0.0, 0, false);
+        if (mv.kind == MC_SWAP) {
+          const int ta = pt[mv.a];
+          pt[mv.a] = pt[mv.b];
+          pt[mv.b] = ta;
+        } else {
+          for (int x = 0; x < 3; ++x) sx[3 * mv.a + x] = xold[x];
+        }
+      }
+    }
+    __syncthreads();
+    pass = s_pass != 0;
+  }
+  if (pass) {
+    // the proposal that passed goes to the full potential, applied in place
+    const Move mv = smv;
+    for (int j = threadIdx.x; j < N; j += blockDim.x) {
+      const size_t g = base + j;
+      if (saff[j]) {
+        const FirstStage &S = tab[pt[j] - 1];
+        SDesc G;
+        trial_desc<RN>(m, S, j, mv, sx, pt, xold, Gm, R, G);
+        fs_store(S, m.nsf_s, m.G_trial + g * m.nsf_s, G);
+        m.E_s_trial[g] = sE[j];
+      }
+      m.affected[g] = saff[j];
+    }
+    if (threadIdx.x == 0) {
+      if (mv.kind == MC_SWAP) {
+        m.types[base + mv.a] = pt[mv.a];
+        m.types[base + mv.b] = pt[mv.b];
+      } else {
+        for (int x = 0; x < 3; ++x) {
+          m.old_pos[3 * c + x] = xold[x];
+          m.coords[3 * (base + mv.a) + x] = sx[3 * mv.a + x];
+        }
+      }
+      m.kind[c] = mv.kind;
+      m.site_a[c] = mv.a;
+      m.site_b[c] = mv.b;
+      m.u_stage1[c] = mv.u1;
+      m.u_stage2[c] = mv.u2;
+      m.dE_s[c] = dE_pass;
+      m.need[c] = 1;
+    }
+  } else if (threadIdx.x == 0) {
+    m.kind[c] = MC_NONE;
+    m.need[c] = 0;
+  }
+  if (threadIdx.x == 0) {
+    m.rng[c] = s;
+    log_close(m, c, L);
   }
 }
 
 // one workgroup per chain: the atoms the full potential must re-evaluate
-__global__ __launch_bounds__(1024) void k_mc_active(MCDev m, int plain) {
+__global__ __launch_bounds__(1024) void k_mc_active(MCDev m) {
   __shared__ int wcount[16];
   __shared__ int base_out;
   const int c = blockIdx.x;
@@ -397,9 +710,7 @@ __global__ __launch_bounds__(1024) void k_mc_active(MCDev m, int plain) {
   const int N = m.natoms;
   const size_t base = (size_t)c * N;
   const int kind = m.kind[c];
-  if (plain && j == 0) m.need[c] = kind != MC_NONE ? 1 : 0;
-  __syncthreads();
-  const bool need = plain ? (kind != MC_NONE) : (m.need[c] != 0);
+  const bool need = m.need[c] != 0;
   bool in = false;
   if (need && j < N) {
     const int a = m.site_a[c], b = m.site_b[c];
@@ -470,7 +781,7 @@ __global__ __launch_bounds__(64) void k_mc_finish(MCDev m, int plain) {
         }
       }
     }
-  } else if (lane == 0 && kind != MC_NONE) {
+  } else if (lane == 0 && need) {
     const int a = m.site_a[c], b = m.site_b[c];
     if (kind == MC_SWAP) {
       const int ta = m.types[base + a];
@@ -481,47 +792,45 @@ __global__ __launch_bounds__(64) void k_mc_finish(MCDev m, int plain) {
     }
   }
   if (lane == 0) {
-    if (m.trace) {
-      const long long k = *m.step - 1;
-      if (k >= 0 && k < m.max_trace) {
-        double *tr = m.trace + 4 * ((size_t)k * m.nchains + c);
-        tr[0] = plain ? 0.0 : m.dE_s[c];
-        tr[1] = need ? dE : 0.0;
-        tr[2] = kind == MC_NONE ? -1.0 : (need ? 1.0 : 0.0);
-        tr[3] = accept ? 1.0 : 0.0;
-      }
-    }
     if (accept) m.E_chain[c] += dE;
-    if (kind != MC_NONE) {
-      m.stats[4 * c] += 1;                     // proposed
-      if (accept) m.stats[4 * c + 1] += 1;     // accepted
-      if (need) m.stats[4 * c + 2] += 1;       // full-potential evaluations
-      if (kind == MC_SWAP) m.stats[4 * c + 3] += 1;
-    }
-    const long step = *m.step;
-    if (m.samples && m.sample_every > 0 && step % m.sample_every == 0) {
-      const long k = step / m.sample_every - 1;
-      if (k >= 0 && k < m.max_samples) m.samples[(size_t)k * m.nchains + c] = m.E_chain[c];
+    if (need) {
+      ChainLog L = log_open(m, c);
+      log_proposal(m, c, L, kind, plain ? 0.0 : m.dE_s[c], dE, 1, accept);
+      log_close(m, c, L);
     }
   }
 }
 
+// decision record: every entry (dE_first, dE_full, stage-1 outcome,
+// accepted) = (0, 0, -1, 0) until a proposal fills it
+__global__ void k_mc_trace_init(double *tr, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) tr[i] = (i & 3) == 2 ? -1.0 : 0.0;
+}
+
 // first-stage state from scratch: every atom's descriptor over the whole
-// cell and its energy, with the arithmetic of k_mc_student
-__global__ __launch_bounds__(256) void k_mc_student_init(MCDev m, ModelDev md) {
-  extern __shared__ double sx[];
+// cell and its energy, with the arithmetic of k_mc_screen
+__global__ __launch_bounds__(256) void k_mc_student_init(MCDev m, ModelDev md, const FirstStage *tabg) {
+  extern __shared__ double lds[];
   const int c = blockIdx.x;
   const int N = m.natoms;
   const size_t base = (size_t)c * N;
+  double *sx = lds;
+  FirstStage *tab = reinterpret_cast<FirstStage *>(sx + 3 * N);
+  copy_first_stage(tabg, md.ntypes, tab);
   for (int k = threadIdx.x; k < 3 * N; k += blockDim.x) sx[k] = m.coords[3 * base + k];
   __syncthreads();
   for (int j = threadIdx.x; j < N; j += blockDim.x) {
     const size_t g = base + j;
-    const SpeciesDev &sp = md.species[m.types[g] - 1];
+    const FirstStage &S = tab[m.types[g] - 1];
     SDesc G;
-    student_full(m, sp, sx, j, base, G);
-    sdesc_store(m, sp, m.G_s + g * m.nsf_s, G);
-    m.E_s[g] = student_energy(md, sp, G);
+    sdesc_zero(G);
+    for (int k = 0; k < N; ++k) {
+      if (k == j) continue;
+      radial_terms_f<SR>(S.Rc, S.rN, dist(m, &sx[3 * j], &sx[3 * k]), 1.0f, S.spin[m.types[base + k] - 1], G.r, G.s);
+    }
+    fs_store(S, m.nsf_s, m.G_s + g * m.nsf_s, G);
+    m.E_s[g] = fs_energy<SR, SW>(S, G);
     m.affected[g] = 0;
   }
 }
@@ -884,6 +1193,10 @@ void launch_mc_neighbors(const MCDev &m, const NbDev &nb, double rc_list, int *f
   hipLaunchKernelGGL(k_mc_neighbors, dim3(std::max(grid, 1)), dim3(256), 0, s, m, nb, rc_list * rc_list, flags);
 }
 
+void launch_mc_trace_init(double *trace, long long n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_mc_trace_init, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, trace, n);
+}
+
 void launch_mc_all_active(const MCDev &m, hipStream_t s) {
   const int n = m.nchains * m.natoms;
   hipLaunchKernelGGL(k_mc_all_active, dim3((n + 255) / 256), dim3(256), 0, s, m);
@@ -893,21 +1206,52 @@ void launch_mc_chain_sum(const MCDev &m, hipStream_t s) {
   hipLaunchKernelGGL(k_mc_chain_sum, dim3(m.nchains), dim3(256), 0, s, m);
 }
 
+size_t first_stage_table_bytes(int ntypes) { return ntypes * sizeof(FirstStage); }
+
+void launch_first_stage_table(const ModelDev &student, void *tab, hipStream_t s) {
+  hipLaunchKernelGGL(k_first_stage_table, dim3(1), dim3(256), 0, s, student, static_cast<FirstStage *>(tab));
+}
+
 void launch_mc_student_init(const MCDev &m, const ModelDev &student, hipStream_t s) {
   const int bs = std::min(256, ((m.natoms + WAVE - 1) / WAVE) * WAVE);
-  hipLaunchKernelGGL(k_mc_student_init, dim3(m.nchains), dim3(bs), 3 * (size_t)m.natoms * sizeof(double), s, m,
-                     student);
+  hipLaunchKernelGGL(k_mc_student_init, dim3(m.nchains), dim3(bs),
+                     3 * (size_t)m.natoms * sizeof(double) + student.ntypes * sizeof(FirstStage), s, m, student,
+                     static_cast<const FirstStage *>(m.fs_tab));
+}
+
+template <bool CACHE, int RN, int NH>
+static void launch_screen(const MCDev &m, const ModelDev &student, size_t lds, int bs, hipStream_t s) {
+  hipLaunchKernelGGL((k_mc_screen<CACHE, RN, NH>), dim3(m.nchains), dim3(bs), lds, s, m, student,
+                     static_cast<const FirstStage *>(m.fs_tab));
+}
+
+bool mc_screen_fits(int natoms, int ntypes, int rn) {
+  return screen_lds_bytes(natoms, ntypes, rn, false) <= device_limits().lds_per_block;
 }
 
 void launch_mc_step(const MCDev &m, const ModelDev *student, bool plain, int phase, hipStream_t s) {
   const int C = m.nchains;
   const int bs = ((m.natoms + WAVE - 1) / WAVE) * WAVE;
   if (phase == 0) {
-    hipLaunchKernelGGL(k_mc_propose, dim3((C + 63) / 64), dim3(64), 0, s, m);
-    if (!plain)
-      hipLaunchKernelGGL(k_mc_student, dim3(C), dim3(std::min(256, bs)), 3 * (size_t)m.natoms * sizeof(double), s,
-                         m, *student);
-    hipLaunchKernelGGL(k_mc_active, dim3(C), dim3(bs), 0, s, m, plain ? 1 : 0);
+    if (plain)
+      hipLaunchKernelGGL(k_mc_propose, dim3((C + 63) / 64), dim3(64), 0, s, m);
+    else {
+      // the chain's first-stage state in LDS when two workgroups still fit on a CU
+      const size_t cached = screen_lds_bytes(m.natoms, student->ntypes, m.rn_s, true);
+      const bool cache = 2 * cached <= device_limits().lds_per_cu && cached <= device_limits().lds_per_block;
+      const size_t lds = cache ? cached : screen_lds_bytes(m.natoms, student->ntypes, m.rn_s, false);
+      const bool small = m.rn_s <= 9 && m.nh_s <= 8;  // radial order 8, 8 hidden nodes
+      const int wg = std::min(256, bs);
+      if (cache && small)
+        launch_screen<true, 9, 8>(m, *student, lds, wg, s);
+      else if (cache)
+        launch_screen<true, SR, SW>(m, *student, lds, wg, s);
+      else if (small)
+        launch_screen<false, 9, 8>(m, *student, lds, wg, s);
+      else
+        launch_screen<false, SR, SW>(m, *student, lds, wg, s);
+    }
+    hipLaunchKernelGGL(k_mc_active, dim3(C), dim3(bs), 0, s, m);
   } else {
     hipLaunchKernelGGL(k_mc_finish, dim3(C), dim3(64), 0, s, m, plain ? 1 : 0);
   }

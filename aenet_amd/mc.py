@@ -42,10 +42,13 @@ class MonteCarlo:
             _native.lib().aenet_amd_mc_destroy(h)
             self.handle = None
 
-    def set(self, T=600.0, p_swap=0.5, max_disp=0.075, beta=None):
+    def set(self, T=600.0, p_swap=0.5, max_disp=0.075, beta=None, max_screen=4):
+        """max_screen: first-stage tries per chain in a delayed-acceptance
+        step (the first that passes goes to the full potential)."""
         stat = C.c_int()
         b = beta if beta is not None else 1.0 / (K_B * T)
-        _native.lib().aenet_amd_mc_set(self.handle, float(b), float(p_swap), float(max_disp), C.byref(stat))
+        _native.lib().aenet_amd_mc_set(self.handle, float(b), float(p_swap), float(max_disp), int(max_screen),
+                                       C.byref(stat))
         check_status(stat.value)
 
     def refresh(self):
@@ -54,7 +57,9 @@ class MonteCarlo:
         check_status(stat.value)
 
     def run(self, nsteps, delayed=False, sample_every=0, graph=True):
-        """nsteps moves of every chain; returns the wall time in s."""
+        """nsteps steps (plain: one proposal per chain; delayed: up to
+        max_screen first-stage tries per chain); energies sampled every
+        sample_every proposals of each chain; returns the wall time in s."""
         stat = C.c_int()
         t0 = time.perf_counter()
         _native.lib().aenet_amd_mc_run(self.handle, int(nsteps), int(delayed), int(sample_every), int(graph),
@@ -88,15 +93,16 @@ class MonteCarlo:
             check_status(stat.value)
         return out
 
-    def arm_trace(self, max_steps):
-        """Record every decision of the next run (max_steps steps)."""
+    def arm_trace(self, max_props):
+        """Record the first max_props decisions of every chain in the next runs."""
         n = C.c_longlong()
         stat = C.c_int()
-        _native.lib().aenet_amd_mc_trace(self.handle, int(max_steps), None, C.byref(n), C.byref(stat))
+        _native.lib().aenet_amd_mc_trace(self.handle, int(max_props), None, C.byref(n), C.byref(stat))
         check_status(stat.value)
 
     def trace(self):
-        """(steps, chains, 4): dE_first, dE_full, stage-1 outcome (1/0/-1), accepted."""
+        """(proposal index, chains, 4): dE_first, dE_full, stage-1 outcome
+        (1 passed, 0 rejected, -1 no record), accepted."""
         n = C.c_longlong()
         stat = C.c_int()
         _native.lib().aenet_amd_mc_trace(self.handle, -1, None, C.byref(n), C.byref(stat))

@@ -45,7 +45,7 @@ def pot_files():
 # ---------------------------------------------------------------------------
 
 def _cpu_worker(args):
-    rank, nranks = args
+    rank, nranks, reps = args
     import ctypes as C
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -63,17 +63,22 @@ def _cpu_worker(args):
     L = np.zeros(9)
     lib.refdrv_cart_to_frac(np.ascontiguousarray(s.cell).ravel(), 1, n, np.ascontiguousarray(s.positions).ravel(),
                             frac, L)
-    tn, te, et = C.c_double(), C.c_double(), C.c_double()
-    st = C.c_int()
-    lib.refdrv_timed(L, 1, n, np.ascontiguousarray(s.types, np.int32), frac, rank, nranks, C.byref(tn), C.byref(te),
-                     C.byref(et), C.byref(st))
-    count = len(range(rank, n, nranks))
-    return count, tn.value, te.value
+    t_nb = t_ev = 0.0
+    for _ in range(reps):
+        tn, te, et = C.c_double(), C.c_double(), C.c_double()
+        st = C.c_int()
+        lib.refdrv_timed(L, 1, n, np.ascontiguousarray(s.types, np.int32), frac, rank, nranks, C.byref(tn),
+                         C.byref(te), C.byref(et), C.byref(st))
+        t_nb += tn.value
+        t_ev += te.value
+    count = reps * len(range(rank, n, nranks))
+    return count, t_nb, t_ev
 
 
-def cpu_baseline(procs, stride):
+def cpu_baseline(procs, stride, reps):
     """Reference library on `procs` worker processes; every worker takes
-    the atoms i = rank (mod procs*stride), so procs*stride sets the sample."""
+    the atoms i = rank (mod procs*stride), so procs*stride sets the sample,
+    and evaluates them reps times."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import Reference
     if not Reference.available():
@@ -82,7 +87,7 @@ def cpu_baseline(procs, stride):
     nranks = procs * stride
     t0 = time.time()
     with ctx.Pool(procs) as pool:
-        res = pool.map(_cpu_worker, [(r, nranks) for r in range(procs)])
+        res = pool.map(_cpu_worker, [(r, nranks, reps) for r in range(procs)])
     wall = time.time() - t0
     atoms = sum(r[0] for r in res)
     t_eval = max(r[2] for r in res)
@@ -93,10 +98,10 @@ def cpu_baseline(procs, stride):
         "unit": "atoms/s",
         "cores": procs,
         "kind": "reference",
-        "sample": "%d of the 4000 atoms (every %dth), %d worker processes over disjoint atoms; "
-                  "value = aenet_atomic_energy_and_forces on precomputed lclist neighbor lists "
+        "sample": "%d atom evaluations: every %dth of the 4000 atoms, %d passes, %d worker processes over "
+                  "disjoint atoms; value = aenet_atomic_energy_and_forces on precomputed lclist neighbor lists "
                   "(kernel-equivalent); end_to_end includes lcl_init + lcl_nbdist_cart as predict.x runs them"
-                  % (atoms, stride, procs),
+                  % (atoms, stride, reps, procs),
         "end_to_end": atoms / t_e2e,
         "cpu_seconds": cpu_s,
         "wall_seconds": wall,
@@ -227,7 +232,7 @@ def mc_throughput(steps, burn):
            "steps": steps, "burn_in": burn}
     for name, first, delayed in (("plain", None, False), ("delayed", student, True)):
         mc = MonteCarlo(teacher, cells, first=first, seed=7)
-        mc.set(T=600.0)
+        mc.set(T=600.0, max_screen=4)
         mc.run(burn)
         mc.stats(reset=True)
         t = mc.run(steps, delayed=delayed)
@@ -235,7 +240,11 @@ def mc_throughput(steps, burn):
         out[name] = {"accepted_moves_per_s": st["accepted"] / t, "proposals_per_s": st["proposed"] / t,
                      "acceptance": st["accepted"] / st["proposed"],
                      "full_potential_fraction": st["full_evaluations"] / st["proposed"],
+                     "second_stage_acceptance": st["accepted"] / st["full_evaluations"],
+                     "proposals_per_chain_step": st["proposed"] / (len(cells) * steps),
                      "ms_per_step": t / steps * 1e3}
+        if delayed:
+            out[name]["max_screen"] = 4
         del mc
     out["delayed_over_plain"] = out["delayed"]["accepted_moves_per_s"] / out["plain"]["accepted_moves_per_s"]
     return out
@@ -250,6 +259,7 @@ def main():
                     help="independent 4000-atom cells per step (default 1: the single alloy cell)")
     ap.add_argument("--cpu-procs", type=int, default=0, help="CPU baseline worker processes (0: host cores, <=16)")
     ap.add_argument("--cpu-stride", type=int, default=1, help="sample = every (procs*stride)th atom")
+    ap.add_argument("--cpu-passes", type=int, default=2, help="CPU baseline passes over the sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched Behler-Parrinello measurement")
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo measurement")
@@ -276,7 +286,7 @@ def main():
     if not args.no_cpu_baseline:
         procs = args.cpu_procs or min(16, sys_cores())
         try:
-            cpu = cpu_baseline(procs, args.cpu_stride)
+            cpu = cpu_baseline(procs, args.cpu_stride, args.cpu_passes)
         except Exception as e:  # reference not built: report, do not fake
             cpu = {"value": None, "unit": "atoms/s", "cores": procs, "kind": "reference",
                    "sample": "unavailable: %s" % e}

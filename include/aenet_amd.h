@@ -176,7 +176,11 @@ void aenet_amd_model_check(aenet_amd_model *model, void *stream, int *stat);
  * stage's energy change (evaluated incrementally, no neighbor search), and
  * only those that pass are decided by the full potential with
  * min(1, exp(-beta (dE_full - dE_first))), which keeps the chain exact for
- * the full potential.  The full potential re-evaluates only the atoms
+ * the full potential.  In a delayed-acceptance step each chain screens up
+ * to max_screen proposals and sends the first that passes to the full
+ * potential; those rejected by the first stage are complete steps of the
+ * chain, so every chain follows sequential delayed acceptance while the
+ * full potential gets a proposal from nearly every chain.  The full potential re-evaluates only the atoms
  * within its cutoff of the moved sites.  Each step's kernels can be
  * captured in a HIP graph and replayed.  No counterpart in the
  * reference. */
@@ -187,27 +191,34 @@ void aenet_amd_mc_create(const aenet_amd_model *full, const aenet_amd_model *fir
                          const double coords[], unsigned long long seed,
                          aenet_amd_mc **mc, int *stat);
 void aenet_amd_mc_destroy(aenet_amd_mc *mc);
+/* max_screen: first-stage tries per chain and delayed-acceptance step
+ * (1 .. 1024; 1 screens one proposal per step) */
 void aenet_amd_mc_set(aenet_amd_mc *mc, double beta, double p_swap, double max_disp,
-                      int *stat);
+                      int max_screen, int *stat);
 /* energies (and first-stage descriptors) recomputed from scratch */
 void aenet_amd_mc_refresh(aenet_amd_mc *mc, int *stat);
-/* nsteps moves of every chain (synchronous); delayed != 0: two-stage;
- * sample_every > 0 records every chain's energy each sample_every steps */
+/* nsteps steps (synchronous): plain, one proposal per chain and step;
+ * delayed != 0, two-stage, up to max_screen proposals per chain and step.
+ * sample_every > 0 records every chain's energy after each sample_every-th
+ * proposal of that chain (chains of a delayed run advance at different
+ * rates) */
 void aenet_amd_mc_run(aenet_amd_mc *mc, int nsteps, int delayed, int sample_every,
                       int use_graph, int *stat);
 /* sums over chains: proposed, accepted, full-potential evaluations, swaps
  * proposed; reset != 0 clears them */
 void aenet_amd_mc_stats(aenet_amd_mc *mc, long long out[4], int reset, int *stat);
 void aenet_amd_mc_energies(aenet_amd_mc *mc, double E_chain[], int *stat);
-/* out (may be NULL): nsamples*nchains energies of the last run */
+/* out (may be NULL): nsamples*nchains energies of the last run, nsamples =
+ * the samples every chain completed */
 void aenet_amd_mc_samples(aenet_amd_mc *mc, double out[], long long *nsamples, int *stat);
 void aenet_amd_mc_state(aenet_amd_mc *mc, double coords[], int types[], int *stat);
-/* Decision record: max_steps >= 0 arms it for the next runs (steps
- * counted from the start of each run; 0 disarms); max_steps < 0 copies it
- * into out (may be NULL): per step and chain 4 doubles -- first-stage
- * energy change, full-potential change (0 if not evaluated), stage-1
- * outcome (1 passed, 0 rejected, -1 no move), accepted. */
-void aenet_amd_mc_trace(aenet_amd_mc *mc, long long max_steps, double out[], long long *nsteps,
+/* Decision record: max_props >= 0 arms it for the next runs (each chain's
+ * proposals counted from the start of each run; 0 disarms); max_props < 0
+ * copies it into out (may be NULL): per proposal index and chain 4
+ * doubles -- first-stage energy change, full-potential change (0 if not
+ * evaluated), stage-1 outcome (1 passed, 0 rejected, -1 no record),
+ * accepted. */
+void aenet_amd_mc_trace(aenet_amd_mc *mc, long long max_props, double out[], long long *nprops,
                         int *stat);
 /* first-stage atomic energies and unscaled descriptors (nsf per atom) */
 void aenet_amd_mc_first_stage_state(aenet_amd_mc *mc, double E_atoms[], double G[], int *nsf,
