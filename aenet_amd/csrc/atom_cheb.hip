@@ -212,7 +212,7 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
     const int ntrip = na * (na - 1) / 2;
     for (int p = lane; p < ntrip; p += WAVE) {
       int a, c;
-      tri_decode(p, na, a, c);
+      tri_decode_diag(p, na, a, c);
       int qj = S.ang[a], qk = S.ang[c];
       const float4 Aj = ang_get(a, qj), Ak = ang_get(c, qk);
       float cosv = (S.R[3 * qj] * S.R[3 * qk] + S.R[3 * qj + 1] * S.R[3 * qk + 1] + S.R[3 * qj + 2] * S.R[3 * qk + 2]) *
@@ -274,6 +274,22 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
     float *fs = S.fs;
     const float *cr1 = S.net.coef + sp.r_i1, *cr2 = S.net.coef + sp.r_i2;
     const float *ca1 = S.net.coef + sp.a_i1, *ca2 = S.net.coef + sp.a_i2;
+    // dE/dG of the radial terms, plain (p) and typespin (s): in registers
+    // up to 19 terms, read from LDS beyond (register budget)
+    constexpr bool RREG = RN <= 19;
+    constexpr int RR = RREG ? RN : 1;
+    float crp_[RR], crs_[RR];
+#pragma unroll
+    for (int k = 0; k < RR; ++k) {
+      crp_[k] = k < rN ? cr1[k] : 0.0f;
+      crs_[k] = (multi && k < rN) ? cr2[k] : 0.0f;
+    }
+    auto rcoef = [&](int k, float sj) -> float {
+      if constexpr (RREG)
+        return crp_[k] + sj * crs_[k];
+      else
+        return k < rN ? cr1[k] + sj * (multi ? cr2[k] : 0.0f) : 0.0f;
+    };
     for (int q = lane; q < n; q += WAVE) {
       float gx = 0.0f, gy = 0.0f, gz = 0.0f;
       float r = S.r[q];
@@ -284,14 +300,13 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
         float sj = multi ? sp.spin[S.t[q] - 1] : 0.0f;
         float fcx = fc * 2.0f * inv_rRc;
         // sigma = sum_n a_n (dfc T_n + fc T'_n dx/dr), T'_n = n U_{n-1}
-        float sig = (cr1[0] + sj * cr2[0]) * dfc;
+        float sig = rcoef(0, sj) * dfc;
         float Tm = 1.0f, T = x, Um = 0.0f, U = 1.0f;
 #pragma unroll
         for (int k = 1; k < RN; ++k) {
-          if (k < rN) {
-            float a = cr1[k] + sj * cr2[k];
-            sig += a * (dfc * T + fcx * (float)k * U);
-          }
+          // zero coefficients beyond the species' rN
+          const float a = rcoef(k, sj);
+          sig += a * (dfc * T + fcx * (float)k * U);
           float Tn = 2.0f * x * T - Tm;
           float Un = 2.0f * x * U - Um;
           Tm = T;
@@ -318,7 +333,7 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
     }
     for (int p = lane; p < ntrip; p += WAVE) {
       int a, c;
-      tri_decode(p, na, a, c);
+      tri_decode_diag(p, na, a, c);
       int qj = S.ang[a], qk = S.ang[c];
       const float4 Aj = ang_get(a, qj), Ak = ang_get(c, qk);
       const float ij = Aj.x, ik = Ak.x;

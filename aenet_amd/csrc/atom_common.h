@@ -55,71 +55,79 @@ __device__ __forceinline__ void cutoff(float r, float Rc, float inv_Rc, float &f
 __device__ __forceinline__ void tri_decode(int p, int n, int &a, int &b) {
   float nn = (float)(2 * n - 1);
   int r = (int)((nn - sqrtf(nn * nn - 8.0f * (float)p)) * 0.5f);
-  if (r < 0) r = 0;
-  if (r > n - 2) r = n - 2;
-  while (r > 0 && r * n - r * (r + 1) / 2 > p) --r;
-  while ((r + 1) * n - (r + 1) * (r + 2) / 2 <= p) ++r;
+  r = min(max(r, 0), n - 2);
+  // the FP32 estimate is within one row of the exact one (n <= 512)
+  if (r > 0 && r * n - r * (r + 1) / 2 > p) --r;
+  if (r < n - 2 && (r + 1) * n - (r + 1) * (r + 2) / 2 <= p) ++r;
   a = r;
   b = p - (r * n - r * (r + 1) / 2) + r + 1;
+}
+
+// triangular index p -> (a, b), 0 <= a < b < n, by diagonals: d = b - a
+// outer (1 .. n-1), a inner.  Consecutive indexes of one diagonal have
+// distinct a and distinct b, so the lanes of a wavefront that scatter into
+// per-neighbor sums seldom hit the same address.
+__device__ __forceinline__ void tri_decode_diag(int p, int n, int &a, int &b) {
+  int r, c;
+  tri_decode(p, n, r, c);  // row r of the row-major order = diagonal r + 1
+  a = c - r - 1;
+  b = c;
 }
 
 // Reduce-scatter of V per-lane values (V a power of two <= 64) over the
 // wavefront: on return lane l holds the sum over all 64 lanes of
 // v[l >> (6 - log2 V)] (64/V consecutive lanes hold the same feature).
+// Level O (32, 16, .., 64/V) halves the values a lane holds: the lanes with
+// bit O clear keep the lower half, the others the upper half, each adding
+// its butterfly partner's copy (permlane swaps for O = 32, 16; DPP below);
+// the remaining levels sum the copies of one feature.
+template <int O, int H, class T>
+__device__ __forceinline__ void rs_level(T (&v)[2 * H]) {
+  if constexpr (O >= 16) {
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      T a = v[k], b = v[k + H];
+      half_swap<O>(a, b);
+      v[k] = a + b;
+    }
+  } else {
+    const bool up = lane_id() & O;
+#pragma unroll
+    for (int k = 0; k < H; ++k) {
+      const T lo = v[k], hi = v[k + H];
+      const T send = up ? lo : hi;
+      const T keep = up ? hi : lo;
+      v[k] = keep + partner<O>(send);
+    }
+  }
+}
+
+template <int O, int V, class T>
+__device__ __forceinline__ T rs_levels(T (&v)[V]) {
+  if constexpr (V == 1) {
+    return butterfly_sum<O>(v[0]);
+  } else {
+    rs_level<O, V / 2>(v);
+    T (&w)[V / 2] = *reinterpret_cast<T (*)[V / 2]>(&v[0]);
+    return rs_levels<O / 2, V / 2>(w);
+  }
+}
+
 template <int V>
 __device__ __forceinline__ double reduce_scatter(double (&v)[V]) {
   static_assert(V >= 1 && V <= 64 && (V & (V - 1)) == 0, "power of two up to 64");
-  const int lane = lane_id();
-  int off = 32;
-#pragma unroll
-  for (int half = V / 2; half >= 1; half >>= 1) {
-    const bool up = lane & off;
-#pragma unroll
-    for (int k = 0; k < half; ++k) {
-      const double lo = v[k], hi = v[k + half];
-      const double send = up ? lo : hi;
-      const double keep = up ? hi : lo;
-      v[k] = keep + __shfl_xor(send, off, WAVE);
-    }
-    off >>= 1;
-  }
-  double s = v[0];
-#pragma unroll
-  for (int o = 32 / V; o >= 1; o >>= 1) s += __shfl_xor(s, o, WAVE);
-  return s;
+  return rs_levels<32, V>(v);
 }
 
 // the same for FP32 partial sums: the first level in FP32, the rest FP64
 template <int V>
 __device__ __forceinline__ double reduce_scatter(float (&v)[V]) {
   static_assert(V >= 2 && V <= 64 && (V & (V - 1)) == 0, "power of two 2..64");
-  const int lane = lane_id();
-  const bool up = lane & 32;
+  rs_level<32, V / 2>(v);
   double d[V / 2];
 #pragma unroll
-  for (int k = 0; k < V / 2; ++k) {
-    const float lo = v[k], hi = v[k + V / 2];
-    const float send = up ? lo : hi;
-    const float keep = up ? hi : lo;
-    d[k] = (double)(keep + __shfl_xor(send, 32, WAVE));
-  }
-  int off = 16;
-#pragma unroll
-  for (int half = V / 4; half >= 1; half >>= 1) {
-    const bool u = lane & off;
-#pragma unroll
-    for (int k = 0; k < half; ++k) {
-      const double lo = d[k], hi = d[k + half];
-      const double send = u ? lo : hi;
-      const double keep = u ? hi : lo;
-      d[k] = keep + __shfl_xor(send, off, WAVE);
-    }
-    off >>= 1;
-  }
-  double s = d[0];
-#pragma unroll
-  for (int o = 32 / V; o >= 1; o >>= 1) s += __shfl_xor(s, o, WAVE);
-  return s;
+  for (int k = 0; k < V / 2; ++k) d[k] = (double)v[k];
+  return rs_levels<16, V / 2>(d);
 }
 
 // LDS of the network stage of one wavefront

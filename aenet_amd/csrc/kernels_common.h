@@ -27,15 +27,87 @@ __device__ __forceinline__ T uniform(T v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
-__device__ __forceinline__ float wave_sum(float v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+// Lane exchanges without the LDS crossbar (gfx950).
+// half_swap<O>(a, b), O = 32 or 16: v_permlane32_swap / v_permlane16_swap
+// exchange the upper O lanes of every 2*O-lane group of a with the lower O
+// lanes of b, so that afterwards a + b holds, in the lower half of each
+// group, the sum of a over both halves and, in the upper half, the sum of b
+// over both halves (one step of a reduce-scatter; with a = b = x the pair
+// sum of x in every lane).
+// partner<O>(x), O = 8, 4, 2, 1: x of the butterfly partner within the
+// lane's row by DPP -- l ^ 8 (row_ror:8), the mirror l ^ 7 in the group of
+// 8 (row_half_mirror), l ^ 2 and l ^ 1 (quad_perm).  The partner differs
+// from l in bit O and agrees with it in the bits above O.
+__device__ __forceinline__ void half_swap_u32(unsigned &a, unsigned &b, int o) {
+  if (o == 32) {
+    const auto r = __builtin_amdgcn_permlane32_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+  } else {
+    const auto r = __builtin_amdgcn_permlane16_swap(a, b, false, false);
+    a = r[0];
+    b = r[1];
+  }
+}
+
+template <int O>
+__device__ __forceinline__ void half_swap(float &a, float &b) {
+  static_assert(O == 32 || O == 16, "permlane swaps");
+  unsigned x = __float_as_uint(a), y = __float_as_uint(b);
+  half_swap_u32(x, y, O);
+  a = __uint_as_float(x);
+  b = __uint_as_float(y);
+}
+
+template <int O>
+__device__ __forceinline__ void half_swap(double &a, double &b) {
+  static_assert(O == 32 || O == 16, "permlane swaps");
+  const unsigned long long x = (unsigned long long)__double_as_longlong(a);
+  const unsigned long long y = (unsigned long long)__double_as_longlong(b);
+  unsigned xl = (unsigned)x, xh = (unsigned)(x >> 32), yl = (unsigned)y, yh = (unsigned)(y >> 32);
+  half_swap_u32(xl, yl, O);
+  half_swap_u32(xh, yh, O);
+  a = __longlong_as_double((long long)(((unsigned long long)xh << 32) | xl));
+  b = __longlong_as_double((long long)(((unsigned long long)yh << 32) | yl));
+}
+
+template <int O>
+__device__ __forceinline__ unsigned partner_u32(unsigned x) {
+  static_assert(O == 8 || O == 4 || O == 2 || O == 1, "DPP within a row");
+  constexpr int ctrl = O == 8 ? 0x128 : O == 4 ? 0x141 : O == 2 ? 0x4E : 0xB1;
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, ctrl, 0xf, 0xf, false);
+}
+
+template <int O>
+__device__ __forceinline__ float partner(float x) {
+  return __uint_as_float(partner_u32<O>(__float_as_uint(x)));
+}
+
+template <int O>
+__device__ __forceinline__ double partner(double x) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = partner_u32<O>((unsigned)u), hi = partner_u32<O>((unsigned)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// sum over the lanes of the butterfly levels O, O/2, .., 1 (every lane
+// ends with the same value: each step adds two equal operand pairs)
+template <int O, class T>
+__device__ __forceinline__ T butterfly_sum(T v) {
+  if constexpr (O >= 16) {
+    T a = v, b = v;
+    half_swap<O>(a, b);
+    v = a + b;
+  } else if constexpr (O >= 1) {
+    v += partner<O>(v);
+  }
+  if constexpr (O > 1) return butterfly_sum<O / 2>(v);
   return v;
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
-  return v;
-}
+__device__ __forceinline__ float wave_sum(float v) { return butterfly_sum<32>(v); }
+
+__device__ __forceinline__ double wave_sum(double v) { return butterfly_sum<32>(v); }
 
 // inclusive prefix sum over the wavefront
 __device__ __forceinline__ int wave_scan(int x) {
