@@ -94,6 +94,9 @@ struct AtomSizes {
   // active[0 .. *nactive-1]; natoms is then the largest possible count
   const int *active;
   const int *nactive;
+  // launch shape chosen by measurement (0 / -1: the occupancy model's)
+  int tune_waves;    // wavefronts per workgroup
+  int tune_staged;   // network weights staged in LDS (0 / 1)
 };
 
 // network training (train.hip): per-atom normalized descriptors in, atomic

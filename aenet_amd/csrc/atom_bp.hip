@@ -70,6 +70,7 @@ struct BPLaunch {
   int hs;        // network layer stride
   const int *active;   // optional subset of central atoms (mode 0)
   const int *nactive;
+  int tune_waves, tune_staged;  // host side: measured launch shape
 };
 
 __host__ __device__ inline int bp_wave_words(const BPLaunch &P) {
@@ -605,7 +606,8 @@ static void launch_bp(const ModelDev &md, const BatchDev &b, const NbDev &nb, co
                       BPLaunch P, hipStream_t stream) {
   PersistentShape ps;
   const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
-  if (!persistent_shape(wbytes, 4 * (size_t)bp_wave_words(P), P.natoms, 3, ps)) {
+  if (!persistent_shape(wbytes, 4 * (size_t)bp_wave_words(P), P.natoms, 3, ps, ATOM_MAX_WAVES, P.tune_waves,
+                        P.tune_staged)) {
     raise_flag(b.flags, AENET_AMD_FLAG_NB_OVERFLOW, stream);
     return;
   }
@@ -614,9 +616,7 @@ static void launch_bp(const ModelDev &md, const BatchDev &b, const NbDev &nb, co
   hipLaunchKernelGGL((k_atom_bp<MODE, F>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream, md, b, nb, o, e, P);
 }
 
-void launch_atom_bp(int mode, bool forces, const AtomSizes &z, const ModelDev &md, const BatchDev &b,
-                    const NbDev &nb, const OutDev &o, const EnvDev &e, hipStream_t stream) {
-  if (z.natoms <= 0) return;
+static BPLaunch bp_launch(int mode, const AtomSizes &z) {
   BPLaunch P;
   P.natoms = z.natoms;
   P.cap = z.cap;
@@ -628,6 +628,23 @@ void launch_atom_bp(int mode, bool forces, const AtomSizes &z, const ModelDev &m
   P.waves = 1;
   P.active = mode == 0 ? z.active : nullptr;
   P.nactive = mode == 0 ? z.nactive : nullptr;
+  P.tune_waves = z.tune_waves;
+  P.tune_staged = z.tune_staged;
+  return P;
+}
+
+bool atom_bp_shape_ok(const AtomSizes &z) {
+  const BPLaunch P = bp_launch(0, z);
+  PersistentShape ps;
+  const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
+  return persistent_shape(wbytes, 4 * (size_t)bp_wave_words(P), P.natoms, 3, ps, ATOM_MAX_WAVES, P.tune_waves,
+                          P.tune_staged);
+}
+
+void launch_atom_bp(int mode, bool forces, const AtomSizes &z, const ModelDev &md, const BatchDev &b,
+                    const NbDev &nb, const OutDev &o, const EnvDev &e, hipStream_t stream) {
+  if (z.natoms <= 0) return;
+  const BPLaunch P = bp_launch(mode, z);
   if (mode == 0) {
     if (forces)
       launch_bp<0, true>(md, b, nb, o, e, P, stream);

@@ -31,9 +31,6 @@ namespace aenet_amd {
 namespace {
 constexpr float EPS_R = 1.0e-12f;
 constexpr int AUXN = 64;  // angular neighbors whose per-neighbor terms are tabulated in LDS
-// wavefronts per workgroup: up to a CU's worth, so that one staged copy of
-// the network weights serves every wavefront of the CU
-constexpr int CHEB_MAX_WAVES = 16;
 }  // namespace
 
 // Per-wavefront LDS region, laid out after the weights in dynamic shared
@@ -60,6 +57,7 @@ struct ChebLaunch {
   int hs;        // network layer stride
   const int *active;   // optional subset of central atoms (mode 0)
   const int *nactive;
+  int tune_waves, tune_staged;  // host side: measured launch shape
 };
 
 __host__ __device__ inline int wave_lds_words(const ChebLaunch &P) {
@@ -418,7 +416,8 @@ static void launch_one(const ModelDev &md, const BatchDev &b, const NbDev &nb, c
                        ChebLaunch P, hipStream_t stream) {
   PersistentShape ps;
   const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
-  if (!persistent_shape(wbytes, 4 * (size_t)wave_lds_words(P), P.natoms, 4, ps, CHEB_MAX_WAVES)) {
+  if (!persistent_shape(wbytes, 4 * (size_t)wave_lds_words(P), P.natoms, 4, ps, CHEB_MAX_WAVES, P.tune_waves,
+                        P.tune_staged)) {
     // cannot happen within the capacity limits; refuse rather than overrun LDS
     raise_flag(b.flags, AENET_AMD_FLAG_NB_OVERFLOW, stream);
     return;
@@ -445,9 +444,7 @@ static void launch_cheb(int mode, bool forces, const ModelDev &md, const BatchDe
   }
 }
 
-void launch_atom_cheb(int rN, int aN, int mode, bool forces, const AtomSizes &z, const ModelDev &md,
-                      const BatchDev &b, const NbDev &nb, const OutDev &o, const EnvDev &e, hipStream_t stream) {
-  if (z.natoms <= 0) return;
+static ChebLaunch cheb_launch(int mode, const AtomSizes &z) {
   ChebLaunch P;
   P.natoms = z.natoms;
   P.cap = z.cap;
@@ -459,6 +456,23 @@ void launch_atom_cheb(int rN, int aN, int mode, bool forces, const AtomSizes &z,
   P.waves = 1;
   P.active = mode == 0 ? z.active : nullptr;
   P.nactive = mode == 0 ? z.nactive : nullptr;
+  P.tune_waves = z.tune_waves;
+  P.tune_staged = z.tune_staged;
+  return P;
+}
+
+bool atom_cheb_shape_ok(const AtomSizes &z) {
+  const ChebLaunch P = cheb_launch(0, z);
+  PersistentShape ps;
+  const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
+  return persistent_shape(wbytes, 4 * (size_t)wave_lds_words(P), P.natoms, 4, ps, CHEB_MAX_WAVES, P.tune_waves,
+                          P.tune_staged);
+}
+
+void launch_atom_cheb(int rN, int aN, int mode, bool forces, const AtomSizes &z, const ModelDev &md,
+                      const BatchDev &b, const NbDev &nb, const OutDev &o, const EnvDev &e, hipStream_t stream) {
+  if (z.natoms <= 0) return;
+  const ChebLaunch P = cheb_launch(mode, z);
   if (rN <= 9 && aN <= 1)
     launch_cheb<9, 1>(mode, forces, md, b, nb, o, e, P, stream);
   else if (rN <= 17 && aN <= 5)

@@ -14,7 +14,6 @@
 
 namespace aenet_amd {
 
-constexpr int ATOM_MAX_WAVES = 4;  // wavefronts per workgroup of most atom kernels (fewer when LDS is short)
 
 // orders the LDS accesses of one wavefront (lanes exchange data through
 // LDS between phases of one atom)
@@ -253,8 +252,25 @@ inline int resident_waves(size_t wbytes, size_t wave_bytes, int simd_waves, int 
 }
 
 inline bool persistent_shape(size_t wbytes, size_t wave_bytes, int natoms, int simd_waves, PersistentShape &ps,
-                             int max_waves = ATOM_MAX_WAVES) {
+                             int max_waves = ATOM_MAX_WAVES, int force_waves = 0, int force_staged = -1) {
   const DeviceLimits &dl = device_limits();
+  if (force_waves > 0 || force_staged >= 0) {
+    // a measured choice: waves per workgroup and weight staging as given,
+    // as many workgroups per CU as LDS and registers allow
+    const bool staged = force_staged >= 0 ? force_staged != 0 : wbytes > 0;
+    if (staged && (wbytes == 0 || wbytes > 64 * 1024)) return false;
+    const int waves = force_waves > 0 ? force_waves : ATOM_MAX_WAVES;
+    if (waves > max_waves) return false;
+    const size_t wg_bytes = (staged ? wbytes : 0) + (size_t)waves * wave_bytes;
+    if (wg_bytes > dl.lds_per_block) return false;
+    const int per_cu = std::min((int)(dl.lds_per_cu / wg_bytes), std::max(1, 4 * simd_waves / waves));
+    if (per_cu < 1) return false;
+    ps.w_in_lds = staged;
+    ps.waves = waves;
+    ps.lds = wg_bytes;
+    ps.grid = std::max(1, std::min((natoms + waves - 1) / waves, dl.cus * per_cu));
+    return true;
+  }
   int wg_plain = 0, wg_staged = 0;
   // without staging, workgroups of at most ATOM_MAX_WAVES suffice
   const int r_plain = resident_waves(0, wave_bytes, simd_waves, std::min(max_waves, ATOM_MAX_WAVES), wg_plain);

@@ -5,6 +5,7 @@
 // process-global; a mutex serializes calls that use it.
 #include <hip/hip_runtime_api.h>
 
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -909,6 +910,48 @@ void aenet_amd_batch_eval_model(aenet_amd_batch *batch, const aenet_amd_model *m
   hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
   *stat = batch->batch.eval(*dm, d_coords, d_types, d_E_atoms, d_E_struct, d_F, s);
   if (*stat) set_err(*stat, "kernel launch failed");
+}
+
+void aenet_amd_batch_autotune(aenet_amd_batch *batch, const aenet_amd_model *model, const double *d_coords,
+                              const int *d_types, int forces, int repeats, double results[], int *ncand,
+                              int chosen[2], void *stream, int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!batch || !dm) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or no model");
+    return;
+  }
+  if (!d_coords || !d_types || repeats < 1) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_amd_batch_autotune: bad arguments");
+    return;
+  }
+  if (!batch->batch.fits(*dm)) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "model cutoff exceeds the cutoff the batch was created for");
+    return;
+  }
+  hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
+  std::vector<std::array<double, 3>> out;
+  std::string err;
+  *stat = batch->batch.autotune(*dm, d_coords, d_types, forces != 0, repeats, s, out, err);
+  if (*stat) {
+    set_err(*stat, err);
+    return;
+  }
+  if (ncand) *ncand = (int)out.size();
+  double best = 1e300;
+  int bw = 0, bs = -1;
+  for (size_t k = 0; k < out.size(); ++k) {
+    if (results && k < 16)
+      for (int c = 0; c < 3; ++c) results[3 * k + c] = out[k][c];
+    if (out[k][2] < best) {
+      best = out[k][2];
+      bw = (int)out[k][0];
+      bs = (int)out[k][1];
+    }
+  }
+  if (chosen) {
+    chosen[0] = bw;
+    chosen[1] = bs;
+  }
 }
 
 void aenet_amd_batch_eval_active(aenet_amd_batch *batch, const aenet_amd_model *model, const double *d_coords,

@@ -246,10 +246,10 @@ int Batch::eval(const DeviceModel &dm, const double *d_coords, const int *d_type
     (void)hipEventRecord(e0, stream);
   }
   if (dm.desc == DESC_CHEBYSHEV)
-    launch_atom_cheb(dm.max_rN, dm.max_aN, 0, d_F != nullptr, dm.sizes(b.natoms, nb_cap_), md, b, nb, o, e,
+    launch_atom_cheb(dm.max_rN, dm.max_aN, 0, d_F != nullptr, sizes_for(dm, b.natoms), md, b, nb, o, e,
                      stream);
   else
-    launch_atom_bp(0, d_F != nullptr, dm.sizes(b.natoms, nb_cap_), md, b, nb, o, e, stream);
+    launch_atom_bp(0, d_F != nullptr, sizes_for(dm, b.natoms), md, b, nb, o, e, stream);
   if (e0 && e1) {
     (void)hipEventRecord(e1, stream);
     ev_.push_back(e0);
@@ -276,7 +276,7 @@ int Batch::vjp(const DeviceModel &dm, const double *d_coords, const int *d_types
   o.g_atom = d_g;
   EnvDev e;
   std::memset(&e, 0, sizeof(e));
-  const AtomSizes z = dm.sizes(b.natoms, nb_cap_);
+  const AtomSizes z = sizes_for(dm, b.natoms);
   if (dm.desc == DESC_CHEBYSHEV)
     launch_atom_cheb(dm.max_rN, dm.max_aN, 0, true, z, md, b, nb, o, e, stream);
   else
@@ -326,7 +326,7 @@ int Batch::eval_active(const DeviceModel &dm, const double *d_coords, const int 
   o.E_atom = d_E_atom;
   EnvDev e;
   std::memset(&e, 0, sizeof(e));
-  const AtomSizes z = dm.sizes(max_active, nb_cap_, d_active, d_nactive);
+  const AtomSizes z = sizes_for(dm, max_active, d_active, d_nactive);
   if (dm.desc == DESC_CHEBYSHEV)
     launch_atom_cheb(dm.max_rN, dm.max_aN, 0, false, z, md, b, nb, o, e, stream);
   else
@@ -367,10 +367,85 @@ int Batch::descriptors(const DeviceModel &dm, const double *d_coords, const int 
   EnvDev e;
   std::memset(&e, 0, sizeof(e));
   if (dm.desc == DESC_CHEBYSHEV)
-    launch_atom_cheb(dm.max_rN, dm.max_aN, 0, false, dm.sizes(b.natoms, nb_cap_), md, b, nb, o, e, stream);
+    launch_atom_cheb(dm.max_rN, dm.max_aN, 0, false, sizes_for(dm, b.natoms), md, b, nb, o, e, stream);
   else
-    launch_atom_bp(0, false, dm.sizes(b.natoms, nb_cap_), md, b, nb, o, e, stream);
+    launch_atom_bp(0, false, sizes_for(dm, b.natoms), md, b, nb, o, e, stream);
   return hipGetLastError() == hipSuccess ? AENET_STATUS_OK : AENET_STATUS_ERR_DEVICE;
+}
+
+AtomSizes Batch::sizes_for(const DeviceModel &dm, int natoms, const int *active, const int *nactive) const {
+  AtomSizes z = dm.sizes(natoms, nb_cap_, active, nactive);
+  const auto it = tune_.find(&dm);
+  if (it != tune_.end()) {
+    z.tune_waves = it->second.first;
+    z.tune_staged = it->second.second;
+  }
+  return z;
+}
+
+int Batch::autotune(const DeviceModel &dm, const double *d_coords, const int *d_types, bool forces, int repeats,
+                    hipStream_t stream, std::vector<std::array<double, 3>> &out, std::string &err) {
+  out.clear();
+  repeats = std::max(1, std::min(repeats, 100));
+  const int nat = plan_.natoms;
+  DBuf<double> E_struct, F;
+  if (E_struct.alloc(2 * (size_t)plan_.st.size()) || (forces && F.alloc(3 * (size_t)nat))) {
+    err = "device allocation failed";
+    return AENET_STATUS_ERR_MALLOC;
+  }
+  const bool was_timing = timing_;
+  double t0 = 0.0;
+  int c0 = 0;
+  (void)kernel_time(t0, c0);  // drop events of earlier evaluations
+  tune_.erase(&dm);
+  const int max_waves = dm.desc == DESC_CHEBYSHEV ? CHEB_MAX_WAVES : ATOM_MAX_WAVES;
+  double best = 1e300;
+  std::pair<int, int> choice(0, -1);
+  int st = AENET_STATUS_OK;
+  for (int waves = 1; waves <= max_waves && !st; waves *= 2) {
+    for (int staged = 0; staged <= 1 && !st; ++staged) {
+      AtomSizes z = dm.sizes(nat, nb_cap_);
+      z.tune_waves = waves;
+      z.tune_staged = staged;
+      const bool ok = dm.desc == DESC_CHEBYSHEV ? atom_cheb_shape_ok(z) : atom_bp_shape_ok(z);
+      if (!ok) continue;
+      tune_[&dm] = std::make_pair(waves, staged);
+      timing_ = false;
+      st = eval(dm, d_coords, d_types, scratch_E_.p, E_struct.p, forces ? F.p : nullptr, stream);  // warm-up
+      timing_ = true;
+      for (int r = 0; r < repeats && !st; ++r)
+        st = eval(dm, d_coords, d_types, scratch_E_.p, E_struct.p, forces ? F.p : nullptr, stream);
+      double ms = 0.0;
+      int count = 0;
+      if (!st) st = kernel_time(ms, count);
+      if (!st && count > 0) {
+        const double avg = ms / count;
+        out.push_back({(double)waves, (double)staged, avg});
+        if (avg < best) {
+          best = avg;
+          choice = std::make_pair(waves, staged);
+        }
+      }
+    }
+  }
+  timing_ = was_timing;
+  if (st) {
+    tune_.erase(&dm);
+    err = "evaluation failed while tuning";
+    return st;
+  }
+  st = check_flags(stream, err);
+  if (st) {
+    tune_.erase(&dm);
+    return st;
+  }
+  if (out.empty()) {
+    tune_.erase(&dm);
+    err = "no launch shape fits";
+    return AENET_STATUS_ERR_DEVICE;
+  }
+  tune_[&dm] = choice;
+  return AENET_STATUS_OK;
 }
 
 void Batch::set_timing(bool on) { timing_ = on; }

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <array>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -52,7 +54,7 @@ struct DeviceModel {
   int ntypes = 0, desc = 0, max_rN = 0, max_aN = 0, nsf_max = 0, wsize = 0, nl_max = 0, hmax = 1;
   double rc_max = 0.0;
   AtomSizes sizes(int natoms, int cap, const int *active = nullptr, const int *nactive = nullptr) const {
-    return AtomSizes{natoms, cap, nsf_max, wsize, nl_max, hmax, active, nactive};
+    return AtomSizes{natoms, cap, nsf_max, wsize, nl_max, hmax, active, nactive, 0, -1};
   }
   int upload(const PotSet &ps, hipStream_t s);
   ModelDev view() const;
@@ -91,6 +93,13 @@ class Batch {
   int descriptors(const DeviceModel &dm, const double *d_coords, const int *d_types, int scaled,
                   double *d_G, int stride, hipStream_t stream);
   int check_flags(hipStream_t stream, std::string &err);
+  // measured launch shape of the atom kernel for model dm: every feasible
+  // (wavefronts per workgroup, weight staging) is timed over `repeats`
+  // full evaluations (HIP events around the atom kernel) and the fastest
+  // kept for the batch's later evaluations with dm; out receives
+  // (waves, staged, average ms) per candidate
+  int autotune(const DeviceModel &dm, const double *d_coords, const int *d_types, bool forces, int repeats,
+               hipStream_t stream, std::vector<std::array<double, 3>> &out, std::string &err);
   // HIP events around the atom kernel of every evaluation while enabled
   void set_timing(bool on);
   // summed duration (ms) and count of the timed atom-kernel launches since
@@ -109,6 +118,10 @@ class Batch {
 
  private:
   BatchDev view(const double *d_coords, const int *d_types) const;
+  // atom-kernel sizes of dm with the batch's measured launch shape
+  AtomSizes sizes_for(const DeviceModel &dm, int natoms, const int *active = nullptr,
+                      const int *nactive = nullptr) const;
+  std::map<const DeviceModel *, std::pair<int, int>> tune_;  // model -> (waves, staged)
   int allocate(std::string &err);
   PlanHost plan_;
   double rc_max_ = 0.0;

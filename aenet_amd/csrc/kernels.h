@@ -7,6 +7,13 @@
 
 namespace aenet_amd {
 
+// wavefronts per workgroup at most: the Chebyshev atom kernel takes up to
+// a CU's worth, so that one staged copy of the network weights serves
+// every wavefront of the CU; the other atom kernels take up to 4 (fewer
+// when LDS is short)
+constexpr int CHEB_MAX_WAVES = 16;
+constexpr int ATOM_MAX_WAVES = 4;
+
 // kernels.hip
 void launch_fill_i32(int *p, int v, long n, hipStream_t s);
 void launch_fill_f64(double *p, double v, long n, hipStream_t s);
@@ -38,6 +45,10 @@ const DeviceLimits &device_limits();
 // set bits of the batch's error word from the host side of a launch
 void raise_flag(int *flags, int bits, hipStream_t s);
 
+// the launch shape that z asks for (measured choice or the occupancy
+// model's) fits the device
+bool atom_cheb_shape_ok(const AtomSizes &z);
+bool atom_bp_shape_ok(const AtomSizes &z);
 void launch_atom_cheb(int rN, int aN, int mode, bool forces, const AtomSizes &z, const ModelDev &md,
                       const BatchDev &b, const NbDev &nb, const OutDev &o, const EnvDev &e,
                       hipStream_t stream);

@@ -145,6 +145,25 @@ class DeviceBatch:
         self.check()
         return {"nnb": count, "coo": coo, "dist": dist, "list": lst, "type": typ}
 
+    def autotune(self, forces=True, repeats=5, model=None, coords=None, types=None):
+        """Time every feasible launch shape of the atom kernel on this batch
+        (aenet_amd_batch_autotune) and keep the fastest for the later
+        evaluations with the model.  Returns ((waves, staged), rows) with
+        rows = [(waves, staged, average ms), ...]."""
+        coords = self.coords if coords is None else coords
+        types = self.types if types is None else types
+        self._check_inputs(coords, types)
+        res = np.zeros(48)
+        n = C.c_int(0)
+        chosen = (C.c_int * 2)()
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_autotune(self._h, self._model(model), _ptr(coords), _ptr(types), int(forces),
+                                               int(repeats), res.ctypes.data, C.byref(n), chosen, self._stream(),
+                                               C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_autotune")
+        rows = [(int(res[3 * k]), int(res[3 * k + 1]), float(res[3 * k + 2])) for k in range(min(n.value, 16))]
+        return (chosen[0], chosen[1]), rows
+
     def descriptors(self, stride, scaled=False, coords=None, types=None, model=None):
         coords = self.coords if coords is None else coords
         types = self.types if types is None else types

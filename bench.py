@@ -173,6 +173,7 @@ def batched_bp(steps, warmup):
     try:
         structs = synth.batch_uneven(2048)
         db = DeviceBatch(structs)
+        chosen, rows = db.autotune(forces=True, repeats=2)
         for _ in range(warmup):
             db.eval(check=False)
         db.check()
@@ -206,6 +207,9 @@ def batched_bp(steps, warmup):
             "flop_per_launch": flops,
             "achieved_tflops": flops / k_s / 1e12,
             "frac_fp32_peak": flops / k_s / FP32_PEAK,
+            "launch_shape": {"chosen": {"waves_per_workgroup": chosen[0], "weights_in_lds": chosen[1]},
+                             "candidates_us": [{"waves_per_workgroup": w, "weights_in_lds": s, "kernel_us": t * 1e3}
+                                               for (w, s, t) in rows]},
         }
         del db
         return out
@@ -263,6 +267,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched Behler-Parrinello measurement")
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo measurement")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the occupancy model's launch shape instead of the measured one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 5 times (reference kernel for the "
                          "FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -304,6 +310,13 @@ def main():
         structs = synth.alloy_replicas(args.replicas)
     db = DeviceBatch(structs)
     natoms = db.natoms
+    tune = None
+    if not args.no_autotune:
+        # the atom kernel's launch shape by measurement (aenet_amd_batch_autotune)
+        chosen, rows = db.autotune(forces=True, repeats=10)
+        tune = {"chosen": {"waves_per_workgroup": chosen[0], "weights_in_lds": chosen[1]},
+                "candidates_us": [{"waves_per_workgroup": w, "weights_in_lds": s, "kernel_us": t * 1e3}
+                                  for (w, s, t) in rows]}
     for _ in range(args.warmup):
         db.eval(check=False)
     db.check()
@@ -371,6 +384,7 @@ def main():
             "flop_per_launch": flops,
             "pairs": n_p,
             "triplets": n_t,
+            "launch_shape": tune,
         },
         "cpu_baseline": cpu,
     }

@@ -68,6 +68,22 @@ void aenet_amd_batch_set_timing(aenet_amd_batch *batch, int enable);
 void aenet_amd_batch_kernel_time(aenet_amd_batch *batch, double *ms, int *count,
                                  int *stat);
 
+/* Autotuner of the atom kernel's launch shape for this batch and model
+ * (NULL: the potentials of aenet_load_potential): every feasible pair
+ * (wavefronts per workgroup: 1, 2, 4, .. 16 for Chebyshev, .. 4 for
+ * Behler-Parrinello; network weights staged in LDS or read through the
+ * caches) is timed with HIP events over `repeats` full evaluations of the
+ * given device coordinates and types (with forces if forces != 0), and the
+ * fastest is used by the batch's later evaluations with that model.
+ * Without it the launch shape comes from an occupancy model.  results (may
+ * be NULL, room for 16 rows): per candidate (waves, staged, average ms);
+ * *ncand = their number; chosen[0..1] = the kept (waves, staged).
+ * Synchronous.  No counterpart in the reference. */
+void aenet_amd_batch_autotune(aenet_amd_batch *batch, const aenet_amd_model *model,
+                              const double *d_coords, const int *d_types, int forces,
+                              int repeats, double results[], int *ncand, int chosen[2],
+                              void *stream, int *stat);
+
 /* Bit-exact neighbor lists (lcl_nbdist_cart, src/ext/lclist.f90:520-677)
  * of every atom of the batch, device pointers, capacity cap per atom:
  * d_count[natoms], d_coo[natoms*cap*3], d_dist/d_list/d_type[natoms*cap];

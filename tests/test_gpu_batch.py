@@ -146,3 +146,32 @@ def test_graph_capture_replays_match_eager(structures, device):
         del db
     finally:
         ann.release()
+
+
+def test_autotune_keeps_results(device):
+    """The measured launch shape changes nothing but the time: per-atom
+    energies are bitwise those of the occupancy model's shape (one wavefront
+    computes each atom either way), forces equal up to the order of the FP64
+    atomic sums; every candidate was timed and the kept one is the fastest."""
+    import torch
+    from aenet_amd import synth
+    from aenet_amd.core import ANNPotentials
+    from aenet_amd.device import DeviceBatch
+    from conftest import pot_files
+    cases = [(pot_files("alloy", ["A", "B"]), [synth.binary_fcc(3.8, (4, 4, 4), 0.08, 41, 42)]),
+             (pot_files("bp3", ["A", "B", "C"]), synth.batch_uneven(24))]
+    for pots, structs in cases:
+        ann = ANNPotentials(pots)
+        try:
+            db = DeviceBatch(structs, device=device)
+            _, _, E0, F0 = db.eval()
+            E0, F0 = E0.clone(), F0.clone()
+            chosen, rows = db.autotune(forces=True, repeats=2)
+            print("chosen", chosen, "rows", rows)
+            assert rows and chosen in [(w, s) for (w, s, _) in rows]
+            assert min(rows, key=lambda r: r[2])[:2] == chosen
+            _, _, E1, F1 = db.eval()
+            assert torch.equal(E0, E1)
+            assert (F1 - F0).abs().max().item() < 1e-10
+        finally:
+            ann.release()
