@@ -37,15 +37,14 @@ __global__ void k_scale_f64(double *p, double a, long n) {
 // cell (cell_get, src/ext/lclist.f90:833-845)
 // ---------------------------------------------------------------------
 
-// lattice coordinates and cell of atom g (returns the global cell id)
-__device__ __forceinline__ int frac_cell(const BatchDev &b, int g, double *F) {
+// lattice coordinates and cell of atom g of structure S at Cartesian x
+// (returns the global cell id)
+__device__ __forceinline__ int frac_cell_at(const BatchDev &b, int g, const StructDev *S, double x[3], double *F) {
   if (F) {
     F[3 * (size_t)g] = 0.0;
     F[3 * (size_t)g + 1] = 0.0;
     F[3 * (size_t)g + 2] = 0.0;
   }
-  const StructDev *S = b.st + b.atom_struct[g];
-  double x[3] = {b.cart[3 * (size_t)g], b.cart[3 * (size_t)g + 1], b.cart[3 * (size_t)g + 2]};
   double s[3];
   bool bad = false;
   {
@@ -98,6 +97,11 @@ __device__ __forceinline__ int frac_cell(const BatchDev &b, int g, double *F) {
   return cell;
 }
 
+__device__ __forceinline__ int frac_cell(const BatchDev &b, int g, double *F) {
+  double x[3] = {b.cart[3 * (size_t)g], b.cart[3 * (size_t)g + 1], b.cart[3 * (size_t)g + 2]};
+  return frac_cell_at(b, g, b.st + b.atom_struct[g], x, F);
+}
+
 __global__ void k_frac_bin(BatchDev b, double *F) {
   int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= b.natoms) return;
@@ -107,20 +111,46 @@ __global__ void k_frac_bin(BatchDev b, double *F) {
 // the whole binning of a small batch in one workgroup: counts in LDS,
 // exclusive scan, placement, and the order within each cell (descending
 // atom index, as the reference's linked lists) with the lattice
-// coordinates copied into that order
-constexpr int SMALL_ATOMS = 8192, SMALL_CELLS = 3072;
+// coordinates copied into that order.  Each thread takes the atoms
+// tid + u*1024 (u < SMALL_PER_THREAD) and issues all their global loads
+// before the arithmetic; the cell of every atom stays in LDS.
+constexpr int SMALL_ATOMS = 8192, SMALL_CELLS = 3072, SMALL_PER_THREAD = SMALL_ATOMS / 1024;
 __global__ __launch_bounds__(1024) void k_bin_small(BatchDev b, double *F) {
   __shared__ int cnt[SMALL_CELLS + 1];
   __shared__ int fill[SMALL_CELLS];
   __shared__ int slot[SMALL_ATOMS];
+  __shared__ int cel[SMALL_ATOMS];
   __shared__ int wsum[16];
   const int tid = threadIdx.x;
+  const int nat = b.natoms;
   for (int c = tid; c < b.ncells; c += blockDim.x) {
     cnt[c] = 0;
     fill[c] = 0;
   }
-  __syncthreads();
-  for (int g = tid; g < b.natoms; g += blockDim.x) atomicAdd(&cnt[frac_cell(b, g, F)], 1);
+  {
+    int as[SMALL_PER_THREAD];
+    double xs[SMALL_PER_THREAD][3];
+#pragma unroll
+    for (int u = 0; u < SMALL_PER_THREAD; ++u) {
+      const int g = tid + u * 1024;
+      if (g < nat) {
+        as[u] = b.atom_struct[g];
+        xs[u][0] = b.cart[3 * (size_t)g];
+        xs[u][1] = b.cart[3 * (size_t)g + 1];
+        xs[u][2] = b.cart[3 * (size_t)g + 2];
+      }
+    }
+    __syncthreads();  // counters cleared
+#pragma unroll
+    for (int u = 0; u < SMALL_PER_THREAD; ++u) {
+      const int g = tid + u * 1024;
+      if (g < nat) {
+        const int c = frac_cell_at(b, g, b.st + as[u], xs[u], F);
+        cel[g] = c;
+        atomicAdd(&cnt[c], 1);
+      }
+    }
+  }
   __syncthreads();
   // exclusive scan of cnt -> cell_start (global) and cnt (LDS, in place)
   int carry = 0;
@@ -154,22 +184,48 @@ __global__ __launch_bounds__(1024) void k_bin_small(BatchDev b, double *F) {
     b.cell_start[b.ncells] = carry;
   }
   __syncthreads();
-  for (int g = tid; g < b.natoms; g += blockDim.x) {
-    const int c = b.cellof[g];
+  for (int g = tid; g < nat; g += blockDim.x) {
+    const int c = cel[g];
     slot[cnt[c] + atomicAdd(&fill[c], 1)] = g;
   }
   __syncthreads();
-  for (int p = tid; p < b.natoms; p += blockDim.x) {
-    const int v = slot[p];
-    const int c = b.cellof[v];
-    const int start = cnt[c], n = cnt[c + 1] - start;
-    int rank = 0;
-    for (int k = 0; k < n; ++k) rank += slot[start + k] > v;
-    const int q = start + rank;
-    b.cell_atoms[q] = v;
-    b.sorted_s[3 * (size_t)q] = b.s[3 * (size_t)v];
-    b.sorted_s[3 * (size_t)q + 1] = b.s[3 * (size_t)v + 1];
-    b.sorted_s[3 * (size_t)q + 2] = b.s[3 * (size_t)v + 2];
+  // rank within the cell, then the lattice coordinates into that order
+  int vs[SMALL_PER_THREAD], qs[SMALL_PER_THREAD];
+#pragma unroll
+  for (int u = 0; u < SMALL_PER_THREAD; ++u) {
+    const int p = tid + u * 1024;
+    vs[u] = -1;
+    if (p < nat) {
+      const int v = slot[p];
+      const int c = cel[v];
+      const int start = cnt[c], n = cnt[c + 1] - start;
+      // eight independent counts: the LDS reads of a cell issue together
+      int r8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      int k = 0;
+      for (; k + 8 <= n; k += 8) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) r8[e] += slot[start + k + e] > v;
+      }
+      for (; k < n; ++k) r8[0] += slot[start + k] > v;
+      const int rank = ((r8[0] + r8[1]) + (r8[2] + r8[3])) + ((r8[4] + r8[5]) + (r8[6] + r8[7]));
+      vs[u] = v;
+      qs[u] = start + rank;
+    }
+  }
+  double ss[SMALL_PER_THREAD][3];
+#pragma unroll
+  for (int u = 0; u < SMALL_PER_THREAD; ++u)
+    if (vs[u] >= 0)
+      for (int k = 0; k < 3; ++k) ss[u][k] = b.s[3 * (size_t)vs[u] + k];
+#pragma unroll
+  for (int u = 0; u < SMALL_PER_THREAD; ++u) {
+    if (vs[u] >= 0) {
+      const int q = qs[u];
+      b.cell_atoms[q] = vs[u];
+      b.sorted_s[3 * (size_t)q] = ss[u][0];
+      b.sorted_s[3 * (size_t)q + 1] = ss[u][1];
+      b.sorted_s[3 * (size_t)q + 2] = ss[u][2];
+    }
   }
 }
 

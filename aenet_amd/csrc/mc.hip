@@ -174,6 +174,26 @@ __device__ __forceinline__ float act_f(int f, float z) {
   }
 }
 
+// the activations for the Monte Carlo first stage (a screen: the second
+// stage keeps the chain exact) with the hardware exponential:
+// tanh(z) = sign(z) (1 - e^(-2|z|)) / (1 + e^(-2|z|)), absolute error about
+// 1e-7; the sigmoid likewise
+__device__ __forceinline__ float tanh_fast(float z) {
+  const float t = __expf(-2.0f * fabsf(z));
+  return copysignf(__fdividef(1.0f - t, 1.0f + t), z);
+}
+
+__device__ __forceinline__ float act_fast(int f, float z) {
+  switch (f) {
+    case 0: return z;
+    case 1: return tanh_fast(z);
+    case 2: return __fdividef(1.0f, 1.0f + __expf(-z));
+    case 3: return 1.7159f * tanh_fast(0.666666666666667f * z);
+    case 4: return 1.7159f * tanh_fast(0.666666666666667f * z) + 0.1f * z;
+    default: return 0.0f;
+  }
+}
+
 // first-stage potential of one species in compact form, held in LDS by the
 // Monte Carlo kernels: the radial basis, the normalization of its plain
 // and typespin features, the one-hidden-layer network (weights k-major;
@@ -276,9 +296,9 @@ __device__ __forceinline__ double fs_energy(const Table &S, const SDesc &G) {
     for (int n = 0; n < RN; ++n) z += S.W[k][n] * x[n];
 #pragma unroll
     for (int n = 0; n < RN; ++n) z += S.W[k][SR + n] * x[RN + n];
-    y += S.Wo[k] * act_f(S.fa_h, z);
+    y += S.Wo[k] * act_fast(S.fa_h, z);
   }
-  y = act_f(S.fa_o, y);
+  y = act_fast(S.fa_o, y);
   return S.E_scale * (double)y + S.E_shift + S.E_atom;
 }
 
