@@ -43,6 +43,7 @@ constexpr float LOG2E = 1.44269504088896341f;
 constexpr int RCH = 16;   // radial functions per register chunk
 constexpr int ACH = 32;   // angular functions per register chunk
 constexpr int QCAP = 128; // triplet queue entries per wavefront
+constexpr int BP_SIMD_WAVES = 4;  // wavefronts per SIMD the kernel's registers allow (<= 128 VGPRs)
 constexpr int MT = AENET_AMD_MAX_TYPES;
 
 }  // namespace
@@ -217,12 +218,14 @@ __device__ __forceinline__ void trip_cut(const Trip &T, float rc, TripCut &c) {
   cutoff(T.rjk, rc, inv, c.cjk, c.dcjk);
 }
 
-// energy pass of an angular chunk: G_f += 2 arg^zeta exp(-eta S) C
+// energy pass of an angular chunk: G_f += 2 arg^zeta exp(-eta S) C; each
+// lane sums its own triplets of a block (a few tens) in FP32, the
+// reduce-scatter over the lanes continues in FP64
 struct AngValueOp {
   const float *R, *r;
   const BPFunc *fn;
   int nf;  // functions in this chunk
-  double acc[ACH];
+  float acc[ACH];
   __device__ __forceinline__ void operator()(int ja, int kb, bool valid) {
     if (!valid) return;
     const Trip T = make_trip(R, r, ja, kb);
@@ -251,7 +254,7 @@ struct AngValueOp {
         const bool g4 = f.kind == 4;
         const float P = 2.0f * exp2f((zeta - 1.0f) * L - eta * LOG2E * (g4 ? S3 : S2));
         const float C = g4 ? c.cj * c.ck * c.cjk : c.cj * c.ck;
-        acc[k] += (double)(P * arg * C);
+        acc[k] += P * arg * C;
       }
     }
   }
@@ -486,7 +489,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
           op.fn = fn + c0;
           op.nf = min(ACH, nf - c0);
 #pragma unroll
-          for (int k = 0; k < ACH; ++k) op.acc[k] = 0.0;
+          for (int k = 0; k < ACH; ++k) op.acc[k] = 0.0f;
           for_triplets(S, b2, m2, b3, m3, t2 == t3, bp.ang_g5[key3] != 0, rc * rc, op);
           const double G = reduce_scatter<ACH>(op.acc);
           const int idx = lane >> 1;
@@ -606,7 +609,7 @@ static void launch_bp(const ModelDev &md, const BatchDev &b, const NbDev &nb, co
                       BPLaunch P, hipStream_t stream) {
   PersistentShape ps;
   const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
-  if (!persistent_shape(wbytes, 4 * (size_t)bp_wave_words(P), P.natoms, 3, ps, ATOM_MAX_WAVES, P.tune_waves,
+  if (!persistent_shape(wbytes, 4 * (size_t)bp_wave_words(P), P.natoms, BP_SIMD_WAVES, ps, ATOM_MAX_WAVES, P.tune_waves,
                         P.tune_staged)) {
     raise_flag(b.flags, AENET_AMD_FLAG_NB_OVERFLOW, stream);
     return;
@@ -637,7 +640,7 @@ bool atom_bp_shape_ok(const AtomSizes &z) {
   const BPLaunch P = bp_launch(0, z);
   PersistentShape ps;
   const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
-  return persistent_shape(wbytes, 4 * (size_t)bp_wave_words(P), P.natoms, 3, ps, ATOM_MAX_WAVES, P.tune_waves,
+  return persistent_shape(wbytes, 4 * (size_t)bp_wave_words(P), P.natoms, BP_SIMD_WAVES, ps, ATOM_MAX_WAVES, P.tune_waves,
                           P.tune_staged);
 }
 
