@@ -73,6 +73,7 @@ struct StructDev {
   double Rc2;        // (Rc_max + 1e-3)^2
   double rc_list;    // Rc_max + 1e-3
   double cell_rad;   // circumradius of one grid cell (all cells of a structure alike)
+  double cell_gap[3];  // spacing of the grid's planes along each lattice direction
   int pbc;
   int atom0, natoms;
   int nc[3];
@@ -148,6 +149,9 @@ struct BatchDev {
   const int *combos;          // [2*ncells*combo_cap] pairs (cell, image rank), visiting order
   int combo_cap;
   int kept_cap;               // largest combination count of a home cell (search LDS size)
+  int combo_max;              // largest combination count of a home cell, unclamped (cell search LDS)
+  int *cell_work;             // [work_cap] cell-major search items: home cell or -1
+  int work_cap;               // natoms/8 + ncells + 1
   const int *Tvec;            // all lattice translations
   const int *Cvec;            // all stencil vectors
   const int *self_t;          // self-image ranks

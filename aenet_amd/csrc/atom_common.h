@@ -15,14 +15,6 @@
 namespace aenet_amd {
 
 
-// orders the LDS accesses of one wavefront (lanes exchange data through
-// LDS between phases of one atom)
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 __device__ __forceinline__ void activate(int f, float x, float &y, float &dy) {
   // ff_activate (src/ext/feedforward.f90:932-969)
   switch (f) {

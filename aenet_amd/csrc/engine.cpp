@@ -129,6 +129,8 @@ int Batch::allocate(std::string &err) {
   bad |= combo_count_.alloc(p.ncells) != 0;
   bad |= combos_.alloc(2 * (size_t)p.ncells * combo_cap_) != 0;
   bad |= flags_.alloc(1) != 0;
+  work_cap_ = p.natoms / 8 + p.ncells + 1;
+  bad |= cell_work_.alloc(work_cap_) != 0;
   bad |= scratch_E_.alloc(p.natoms) != 0;
   if (bad) {
     err = "device allocation failed";
@@ -169,6 +171,7 @@ int Batch::allocate(std::string &err) {
     int mx = 1;
     for (int v : cc) mx = std::max(mx, v);
     kept_cap_ = std::min(AENET_AMD_KEPT_CAP, (mx + 15) / 16 * 16);
+    combo_max_ = mx;
   }
   if (hipDeviceSynchronize() != hipSuccess) {
     err = "device error";
@@ -220,6 +223,9 @@ BatchDev Batch::view(const double *d_coords, const int *d_types) const {
   b.combos = combos_.p;
   b.combo_cap = combo_cap_;
   b.kept_cap = kept_cap_;
+  b.combo_max = combo_max_;
+  b.cell_work = cell_work_.p;
+  b.work_cap = work_cap_;
   b.Tvec = Tvec_.p;
   b.Cvec = Cvec_.p;
   b.self_t = self_t_.p;

@@ -1,10 +1,15 @@
 // kernels.hip -- gfx950 kernels of the energy/force path (everything but
 // the descriptor/network/force kernels of atom_cheb.hip / atom_bp.hip).
 //
-// Per step:  k_frac_bin -> k_scan_cells -> k_place -> k_cell_sort ->
-//            k_search (compact lists) -> atom kernel -> k_struct_energy
+// Per step:  k_bin_small (or k_frac_bin -> k_scan_cells -> k_place ->
+//            k_cell_sort) -> k_search_cells (compact lists) -> atom kernel
+//            -> k_struct_energy
+// Subsets of atoms (Monte Carlo, autograd): k_search, a wavefront per atom.
 // Once per batch: k_combos (cell/image combinations of each home cell).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
 
 #include "aenet_amd_internal.h"
 #include "kernels.h"
@@ -29,6 +34,21 @@ __global__ void k_fill_f64(double *p, double v, long n) {
 __global__ void k_scale_f64(double *p, double a, long n) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = a * p[i];
+}
+
+// The cell-major search (k_search_cells) runs one workgroup of CS_WAVES
+// wavefronts per work item (c, j): atoms CS_WAVES*j .. CS_WAVES*j + CS_WAVES-1
+// of home cell c, in cell order.  The binning kernels list the items: item
+// (c, j) sits at slot start_c / CS_WAVES + c + j (start_c = first atom of c in
+// cell order), which is below the next cell's first slot; the slots between
+// are -1.  The slot count is at most natoms / CS_WAVES + ncells + 1.
+constexpr int CS_WAVES = 8;
+
+__device__ inline void write_cell_work(const BatchDev &b, int c, int start, int next, int first, int stride) {
+  const int base = start / CS_WAVES + c;
+  const int end = c + 1 < b.ncells ? next / CS_WAVES + c + 1 : b.work_cap;
+  const int items = (next - start + CS_WAVES - 1) / CS_WAVES;
+  for (int k = base + first; k < end; k += stride) b.cell_work[k] = k - base < items ? c : -1;
 }
 
 // ---------------------------------------------------------------------
@@ -184,6 +204,7 @@ __global__ __launch_bounds__(1024) void k_bin_small(BatchDev b, double *F) {
     b.cell_start[b.ncells] = carry;
   }
   __syncthreads();
+  for (int c = tid; c < b.ncells; c += blockDim.x) write_cell_work(b, c, cnt[c], cnt[c + 1], 0, 1);
   for (int g = tid; g < nat; g += blockDim.x) {
     const int c = cel[g];
     slot[cnt[c] + atomicAdd(&fill[c], 1)] = g;
@@ -279,6 +300,7 @@ __global__ __launch_bounds__(64) void k_cell_sort(BatchDev b, const int *tmp) {
   const int c = blockIdx.x;
   const int start = b.cell_start[c], n = b.cell_start[c + 1] - start;
   if (lane_id() == 0) b.cell_count[c] = 0;  // ready for the next step's k_frac_bin
+  write_cell_work(b, c, start, start + n, lane_id(), WAVE);
   for (int a = lane_id(); a < n; a += WAVE) {
     int v = tmp[start + a];
     int rank = 0;
@@ -294,8 +316,9 @@ __global__ __launch_bounds__(64) void k_cell_sort(BatchDev b, const int *tmp) {
 // (cell, image) combinations of each home cell, in the reference's visiting
 // order (stencil rank major, image rank minor), that can hold a neighbor of
 // an atom of the home cell: cells are parallelepipeds of the structure's
-// grid, bounded by spheres of the grid cell's circumradius.  Computed once
-// per batch; one workgroup of 256 per home cell.
+// grid, bounded by spheres of the grid cell's circumradius and separated by
+// whole plane spacings of the grid.  Computed once per batch; one
+// workgroup of 256 per home cell.
 __global__ __launch_bounds__(256) void k_combos(BatchDev b, int *combo_count, int *combos_out) {
   const int h = blockIdx.x;
   const int tid = threadIdx.x;
@@ -338,12 +361,18 @@ __global__ __launch_bounds__(256) void k_combos(BatchDev b, int *combo_count, in
       if (inside) {
         c = S->cell0 + ((ic[0] - 1) * S->nc[1] + (ic[1] - 1)) * S->nc[2] + (ic[2] - 1);
         double ctr[3] = {b.cell_ctr[3 * c], b.cell_ctr[3 * c + 1], b.cell_ctr[3 * c + 2]};
+        // grid offset of the image cell; cells |d| apart along direction k
+        // are at least (|d| - 1) plane spacings apart
+        int d[3] = {ic[0] - hijk[0], ic[1] - hijk[1], ic[2] - hijk[2]};
         if (t > 0) {
           const int *T = b.Tvec + 3 * (S->T0 + t - 1);
           for (int k = 0; k < 3; ++k) ctr[k] += S->L[k] * T[0] + S->L[3 + k] * T[1] + S->L[6 + k] * T[2];
+          for (int k = 0; k < 3; ++k) d[k] += S->nc[k] * T[k];
         }
+        double gap = 0.0;
+        for (int k = 0; k < 3; ++k) gap = fmax(gap, (double)max(abs(d[k]) - 1, 0) * S->cell_gap[k]);
         double dx = ctr[0] - hc[0], dy = ctr[1] - hc[1], dz = ctr[2] - hc[2];
-        keep = dx * dx + dy * dy + dz * dz <= lim * lim;
+        keep = dx * dx + dy * dy + dz * dz <= lim * lim && gap <= S->rc_list + 1.0e-5;
       }
     }
     unsigned long long m = wave_ballot(keep);
@@ -370,18 +399,42 @@ __global__ __launch_bounds__(256) void k_combos(BatchDev b, int *combo_count, in
   }
 }
 
-// per-structure energy sums in FP64 (fixed reduction order)
+// per-structure energy sums in FP64 (fixed reduction order: thread t sums
+// atoms t, t+256, ... in turn, then a tree over the threads); the loads of
+// SE_UNROLL atoms issue before their sums
+constexpr int SE_UNROLL = 8;
 __global__ __launch_bounds__(256) void k_struct_energy(BatchDev b, ModelDev md, const double *E_atom,
                                                        double *E_tot, double *E_coh) {
   __shared__ double st[256], sc[256];
+  __shared__ double e_free[AENET_AMD_MAX_TYPES];
   const int s = blockIdx.x;
   const StructDev *S = b.st + s;
+  const int n = S->natoms, a0 = S->atom0;
+  if (threadIdx.x < AENET_AMD_MAX_TYPES)
+    e_free[threadIdx.x] = (int)threadIdx.x < md.ntypes ? md.species[threadIdx.x].E_atom : 0.0;
+  __syncthreads();
   double et = 0.0, ec = 0.0;
-  for (int i = threadIdx.x; i < S->natoms; i += 256) {
-    int g = S->atom0 + i;
-    double e = E_atom[g];
-    et += e;
-    ec += e - md.species[b.types[g] - 1].E_atom;
+  for (int i0 = 0; i0 < n; i0 += 256 * SE_UNROLL) {
+    double e[SE_UNROLL];
+    int t[SE_UNROLL];
+#pragma unroll
+    for (int u = 0; u < SE_UNROLL; ++u) {
+      const int i = i0 + u * 256 + (int)threadIdx.x;
+      e[u] = 0.0;
+      t[u] = 1;
+      if (i < n) {
+        e[u] = E_atom[a0 + i];
+        t[u] = b.types[a0 + i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SE_UNROLL; ++u) {
+      const int i = i0 + u * 256 + (int)threadIdx.x;
+      if (i < n) {
+        et += e[u];
+        ec += e[u] - e_free[min(max(t[u], 1), AENET_AMD_MAX_TYPES) - 1];
+      }
+    }
   }
   st[threadIdx.x] = et;
   sc[threadIdx.x] = ec;
@@ -465,6 +518,227 @@ struct ListSink {
   }
 };
 
+// ---------------------------------------------------------------------
+// cell-major search: one workgroup per (home cell, split)
+// ---------------------------------------------------------------------
+// Every atom of a home cell tests the same candidates: the atoms of the
+// home cell's (cell, image) combinations in visiting order.  The workgroup
+// computes their Cartesian coordinates L (s_j + T) with the reference's
+// FP64 expression once, into LDS, and each wavefront scans them for one
+// central atom at a time (distance test only).  The entries that pass come
+// out in the order of search_neighbors: a combination that its per-atom
+// sphere test skips holds no neighbor of that atom, and runs of equal cell
+// are the stencil entries in both (stencil cells are distinct).  Passing
+// candidates queue in LDS and reach the sink 64 at a time.
+constexpr int CS_CHUNK = 1024; // candidates staged at once
+constexpr int CS_QUEUE = 128;  // pending passing candidates per wavefront
+
+struct CellSearchLds {
+  double *cx, *cy, *cz;  // [CS_CHUNK] candidate coordinates (Cartesian, A)
+  int *cj;               // [CS_CHUNK] candidate atom
+  int *queue;            // [CS_WAVES * CS_QUEUE]
+  int *ovf;              // [CS_WAVES] sink overflow word (the count decides)
+  int *hdr;              // [2] groups, candidates
+  int *gk, *gn, *gpos;   // [kc] groups (runs of equal cell): first combination, images, first atom position
+  int *gitems;           // [kc+1] prefix of atoms x images
+};
+
+static inline size_t cell_search_lds_bytes(int kc) {
+  return (size_t)CS_CHUNK * (3 * sizeof(double) + sizeof(int)) +
+         sizeof(int) * ((size_t)CS_WAVES * (CS_QUEUE + 1) + 2 + 4 * (size_t)kc + 1);
+}
+
+__device__ inline CellSearchLds carve_cell_search(double *p, int kc) {
+  CellSearchLds W;
+  W.cx = p;
+  W.cy = p + CS_CHUNK;
+  W.cz = p + 2 * CS_CHUNK;
+  int *q = reinterpret_cast<int *>(p + 3 * CS_CHUNK);
+  W.cj = q;
+  W.queue = W.cj + CS_CHUNK;
+  W.ovf = W.queue + CS_WAVES * CS_QUEUE;
+  W.hdr = W.ovf + CS_WAVES;
+  W.gk = W.hdr + 2;
+  W.gn = W.gk + kc;
+  W.gpos = W.gn + kc;
+  W.gitems = W.gpos + kc;
+  return W;
+}
+
+// Workgroup k takes work item (h, j) of slot k: wavefront w searches for
+// atom a = CS_WAVES*j + w of home cell h.  Candidates are staged chunk by
+// chunk; each wavefront scans a chunk before the next is staged.
+template <bool LISTS>
+__global__ __launch_bounds__(CS_WAVES *WAVE) void k_search_cells(BatchDev b, NbDev nb, OutDev o) {
+  extern __shared__ double cs_dyn[];
+  const int h = b.cell_work[blockIdx.x];
+  if (h < 0) return;  // empty slot (uniform)
+  const int tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
+  const int start = b.cell_start[h], n = b.cell_start[h + 1] - start;
+  const int j = (int)blockIdx.x - (start / CS_WAVES + h);
+  const int kc = b.combo_max;
+  CellSearchLds W = carve_cell_search(cs_dyn, kc);
+  const StructDev *S = b.st + b.cell_struct[h];
+  double L[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) L[k] = S->L[k];
+  const int *Tv = b.Tvec + 3 * S->T0;
+  const double Rc2 = S->Rc2;
+  const int *combos = b.combos + 2 * (size_t)h * b.combo_cap;
+  const int K = min(b.combo_count[h], kc);
+  // groups of the home cell's combinations and their candidate counts
+  if (w == 0) {
+    int ng = 0;
+    for (int base = 0; base < K; base += WAVE) {
+      const int q = base + lane;
+      const int c = q < K ? combos[2 * q] : -1;
+      const bool st = q < K && (q == 0 || combos[2 * (q - 1)] != c);
+      const unsigned long long m = wave_ballot(st);
+      if (st) {
+        const int k = ng + mask_prefix(m);
+        W.gk[k] = q;
+        W.gpos[k] = b.cell_start[c];
+      }
+      ng += popc64(m);
+    }
+    wave_sync();
+    int carry = 0;
+    for (int base = 0; base < ng; base += WAVE) {
+      const int k = base + lane;
+      int items = 0;
+      if (k < ng) {
+        const int q0 = W.gk[k], q1 = k + 1 < ng ? W.gk[k + 1] : K;
+        const int c = combos[2 * q0];
+        W.gn[k] = q1 - q0;
+        items = (b.cell_start[c + 1] - b.cell_start[c]) * (q1 - q0);
+      }
+      const int incl = wave_scan(items);
+      if (k < ng) W.gitems[k] = carry + incl - items;
+      carry += __shfl(incl, WAVE - 1, WAVE);
+    }
+    if (lane == 0) {
+      W.gitems[ng] = carry;
+      W.hdr[0] = ng;
+      W.hdr[1] = carry;
+    }
+  }
+  __syncthreads();
+  const int ng = W.hdr[0], total = W.hdr[1];
+  const int nchunks = max(1, (total + CS_CHUNK - 1) / CS_CHUNK);
+  const int cap = LISTS ? o.nb_cap : nb.cap;
+  int *count_out = LISTS ? o.nb_count : nb.count;
+  int *queue = W.queue + w * CS_QUEUE;
+  const int a = j * CS_WAVES + w;
+  const bool have = a < n;  // uniform per wavefront
+  const int g = have ? b.cell_atoms[start + a] : 0;
+  double si[3] = {0.0, 0.0, 0.0}, coo1[3] = {0.0, 0.0, 0.0};
+  if (have) {
+    si[0] = b.s[3 * (size_t)g];
+    si[1] = b.s[3 * (size_t)g + 1];
+    si[2] = b.s[3 * (size_t)g + 2];
+    matvec_exact(L, si, coo1);
+  }
+
+  // entries of one central atom g among the cn staged candidates, appended
+  // from slot count on; returns the new count
+  auto scan = [&](auto &sink, int g, const double *coo1, int count, int cn) {
+    auto emit = [&](int q, int slot) {
+      const double coo2[3] = {W.cx[q], W.cy[q], W.cz[q]};
+      double cart[3];
+      cart[0] = coo2[0] - coo1[0];
+      cart[1] = coo2[1] - coo1[1];
+      cart[2] = coo2[2] - coo1[2];
+      sink.put(slot, W.cj[q], coo2, cart, sqrt(dist2_exact(cart)));
+    };
+    int qn = 0;
+    for (int base = 0; base < cn; base += WAVE) {
+      const int q = base + lane;
+      bool pass = false;
+      if (q < cn) {
+        double cart[3];
+        cart[0] = W.cx[q] - coo1[0];
+        cart[1] = W.cy[q] - coo1[1];
+        cart[2] = W.cz[q] - coo1[2];
+        pass = W.cj[q] != g && dist2_exact(cart) <= Rc2;
+      }
+      const unsigned long long m = wave_ballot(pass);
+      if (pass) queue[qn + mask_prefix(m)] = q;
+      qn += popc64(m);
+      wave_sync();
+      if (qn >= WAVE) {
+        emit(queue[lane], count + lane);
+        count += WAVE;
+        wave_sync();
+        if (lane < qn - WAVE) queue[lane] = queue[WAVE + lane];
+        qn -= WAVE;
+        wave_sync();
+      }
+    }
+    if (qn > 0) {
+      if (lane < qn) emit(queue[lane], count + lane);
+      count += qn;
+      wave_sync();
+    }
+    return count;
+  };
+
+  auto run = [&](auto &sink) {
+    int count = have ? self_images(b, S, g, L, si, coo1, sink) : 0;
+    for (int ch = 0; ch < nchunks; ++ch) {
+      const int cn = max(0, min(CS_CHUNK, total - ch * CS_CHUNK));
+      __syncthreads();  // the previous chunk's scans are done
+      for (int i = tid; i < cn; i += CS_WAVES * WAVE) {
+        const int q = ch * CS_CHUNK + i;
+        // group of candidate q: last k with gitems[k] <= q
+        int lo = 0, hi = ng - 1;
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (W.gitems[mid] <= q)
+            lo = mid;
+          else
+            hi = mid - 1;
+        }
+        const int off = q - W.gitems[lo], gn = W.gn[lo];
+        const int ai = gn == 1 ? off : off / gn;
+        const int tt = off - ai * gn;
+        const int pos = W.gpos[lo] + ai;
+        const int t = combos[2 * (W.gk[lo] + tt) + 1];
+        const double sj[3] = {b.sorted_s[3 * (size_t)pos], b.sorted_s[3 * (size_t)pos + 1],
+                              b.sorted_s[3 * (size_t)pos + 2]};
+        double coo2[3];
+        if (t == 0) {
+          matvec_exact(L, sj, coo2);
+        } else {
+          double c2[3];
+          add_T_exact(sj, Tv + 3 * (t - 1), c2);
+          matvec_exact(L, c2, coo2);
+        }
+        W.cx[i] = coo2[0];
+        W.cy[i] = coo2[1];
+        W.cz[i] = coo2[2];
+        W.cj[i] = b.cell_atoms[pos];
+      }
+      __syncthreads();
+      if (have) count = scan(sink, g, coo1, count, cn);
+    }
+    if (have && lane == 0) {
+      // a put beyond the capacity was dropped: the count says so
+      if (count > cap) {
+        atomicOr(b.flags, AENET_AMD_FLAG_NB_OVERFLOW);
+        count = cap;
+      }
+      count_out[g] = count;
+    }
+  };
+  if constexpr (LISTS) {
+    ListSink sink{&b, o, g, W.ovf + w};
+    run(sink);
+  } else {
+    CompactSink sink{nb, g, W.ovf + w};
+    run(sink);
+  }
+}
+
 __global__ __launch_bounds__(64) void k_neighbor_lists(BatchDev b, OutDev o) {
   extern __shared__ int sdyn[];
   SearchLds W = carve_search(sdyn, b.kept_cap);
@@ -545,14 +819,27 @@ void launch_binning(const BatchDev &b, int *tmp, double *F, hipStream_t s) {
 
 void launch_search(const BatchDev &b, const NbDev &nb, hipStream_t s, const int *active, const int *nactive,
                    int max_active) {
-  const int n = active ? max_active : b.natoms;
-  if (n > 0)
-    hipLaunchKernelGGL(k_search, dim3(n), dim3(64), 4 * (size_t)search_lds_ints(b.kept_cap), s, b, nb, active, nactive);
+  if (active) {
+    if (max_active > 0)
+      hipLaunchKernelGGL(k_search, dim3(max_active), dim3(64), 4 * (size_t)search_lds_ints(b.kept_cap), s, b, nb,
+                         active, nactive);
+    return;
+  }
+  if (b.natoms > 0 && b.ncells > 0) {
+    OutDev o;
+    memset(&o, 0, sizeof(o));
+    hipLaunchKernelGGL(k_search_cells<false>, dim3(b.work_cap), dim3(CS_WAVES * WAVE),
+                       cell_search_lds_bytes(b.combo_max), s, b, nb, o);
+  }
 }
 
 void launch_neighbor_lists(const BatchDev &b, const OutDev &o, hipStream_t s) {
-  if (b.natoms > 0)
-    hipLaunchKernelGGL(k_neighbor_lists, dim3(b.natoms), dim3(64), 4 * (size_t)search_lds_ints(b.kept_cap), s, b, o);
+  if (b.natoms > 0 && b.ncells > 0) {
+    NbDev nb;
+    memset(&nb, 0, sizeof(nb));
+    hipLaunchKernelGGL(k_search_cells<true>, dim3(b.work_cap), dim3(CS_WAVES * WAVE),
+                       cell_search_lds_bytes(b.combo_max), s, b, nb, o);
+  }
 }
 
 void launch_struct_energy(const BatchDev &b, const ModelDev &md, const double *E_atom, double *E_tot,

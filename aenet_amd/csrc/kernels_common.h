@@ -27,6 +27,14 @@ __device__ __forceinline__ T uniform(T v) {
   return __builtin_amdgcn_readfirstlane(v);
 }
 
+// orders the LDS accesses of one wavefront (lanes exchange data through
+// LDS between phases of one atom)
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Lane exchanges without the LDS crossbar (gfx950).
 // half_swap<O>(a, b), O = 32 or 16: v_permlane32_swap / v_permlane16_swap
 // exchange the upper O lanes of every 2*O-lane group of a with the lower O
@@ -177,22 +185,16 @@ __device__ inline SearchLds carve_search(int *p, int kc) {
 //
 // Sink::put(slot, j, coo2, cart, dist) receives every entry and must check
 // slot against its own capacity.  Returns the number of entries.
+// The periodic images of the central atom g within the cutoff, first in
+// the reference's list (src/ext/lclist.f90:599-624), through sink slots
+// 0..nself-1.  Returns their number.
 template <class Sink>
-__device__ int search_neighbors(const BatchDev &b, int g, Sink &sink, SearchLds &W) {
+__device__ int self_images(const BatchDev &b, const StructDev *S, int g, const double *L, const double *si,
+                           const double *coo1, Sink &sink) {
   const int lane = lane_id();
-  const int sid = uniform(b.atom_struct[g]);
-  const StructDev *S = b.st + sid;
-  double L[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) L[k] = S->L[k];
-  const double si[3] = {b.s[3 * (size_t)g], b.s[3 * (size_t)g + 1], b.s[3 * (size_t)g + 2]};
-  double coo1[3];
-  matvec_exact(L, si, coo1);
-  const double Rc2 = S->Rc2;
   const int *Tv = b.Tvec + 3 * S->T0;
-  int count = 0;
-  // (2) periodic images of the central atom
   const int nself = S->nself;
+  int count = 0;
   for (int base = 0; base < nself; base += WAVE) {
     int q = base + lane;
     if (q < nself) {
@@ -207,6 +209,24 @@ __device__ int search_neighbors(const BatchDev &b, int g, Sink &sink, SearchLds 
     }
     count += min(WAVE, nself - base);
   }
+  return count;
+}
+
+template <class Sink>
+__device__ int search_neighbors(const BatchDev &b, int g, Sink &sink, SearchLds &W) {
+  const int lane = lane_id();
+  const int sid = uniform(b.atom_struct[g]);
+  const StructDev *S = b.st + sid;
+  double L[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) L[k] = S->L[k];
+  const double si[3] = {b.s[3 * (size_t)g], b.s[3 * (size_t)g + 1], b.s[3 * (size_t)g + 2]};
+  double coo1[3];
+  matvec_exact(L, si, coo1);
+  const double Rc2 = S->Rc2;
+  const int *Tv = b.Tvec + 3 * S->T0;
+  // (2) periodic images of the central atom
+  int count = self_images(b, S, g, L, si, coo1, sink);
   // (3) candidates: keep the home cell's (cell, image) combinations that
   // can reach this atom
   const int h = b.cellof[g];

@@ -246,6 +246,18 @@ static int build_plan_impl(double rc_max, int nstruct, const int *natoms, const 
         rad = std::max(rad, 0.5 * std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
       }
       d.cell_rad = rad * (1.0 + 1.0e-12) + 1.0e-9;
+      // plane spacing of family k: cell volume / area of the face spanned
+      // by the other two grid vectors
+      const double *g0 = grid, *g1 = grid + 3, *g2 = grid + 6;
+      const double *gv[3] = {g0, g1, g2};
+      double x01[3] = {g0[1] * g1[2] - g0[2] * g1[1], g0[2] * g1[0] - g0[0] * g1[2], g0[0] * g1[1] - g0[1] * g1[0]};
+      const double vol = std::fabs(x01[0] * g2[0] + x01[1] * g2[1] + x01[2] * g2[2]);
+      for (int k = 0; k < 3; ++k) {
+        const double *u = gv[(k + 1) % 3], *w = gv[(k + 2) % 3];
+        const double c[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+        const double area = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+        d.cell_gap[k] = area > 0.0 ? vol / area : 0.0;
+      }
     }
     plan.ncells += (int)nc;
     std::vector<int> C, T;
