@@ -1,9 +1,9 @@
 // kernels.hip -- gfx950 kernels of the energy/force path (everything but
 // the descriptor/network/force kernels of atom_cheb.hip / atom_bp.hip).
 //
-// Per step:  k_bin_small (or k_frac_bin -> k_scan_cells -> k_place ->
-//            k_cell_sort) -> k_search_cells (compact lists) -> atom kernel
-//            -> k_struct_energy
+// Per step:  k_frac -> k_bin_small (or k_frac_bin -> k_scan_cells ->
+//            k_place -> k_cell_sort) -> k_search_cells (compact lists) ->
+//            atom kernel -> k_struct_energy
 // Subsets of atoms (Monte Carlo, autograd): k_search, a wavefront per atom.
 // Once per batch: k_combos (cell/image combinations of each home cell).
 #include <hip/hip_runtime.h>
@@ -128,18 +128,24 @@ __global__ void k_frac_bin(BatchDev b, double *F) {
   atomicAdd(&b.cell_count[frac_cell(b, g, F)], 1);
 }
 
-// the whole binning of a small batch in one workgroup: counts in LDS,
-// exclusive scan, placement, and the order within each cell (descending
-// atom index, as the reference's linked lists) with the lattice
+// lattice coordinates and cells only (small batches: k_bin_small counts)
+__global__ __launch_bounds__(256) void k_frac(BatchDev b, double *F) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= b.natoms) return;
+  frac_cell(b, g, F);
+}
+
+// the binning of a small batch after k_frac, in one workgroup: counts in
+// LDS, exclusive scan, placement, and the order within each cell
+// (descending atom index, as the reference's linked lists) with the lattice
 // coordinates copied into that order.  Each thread takes the atoms
-// tid + u*1024 (u < SMALL_PER_THREAD) and issues all their global loads
-// before the arithmetic; the cell of every atom stays in LDS.
+// tid + u*1024 (u < SMALL_PER_THREAD) and keeps their cells and lattice
+// coordinates in registers.
 constexpr int SMALL_ATOMS = 8192, SMALL_CELLS = 3072, SMALL_PER_THREAD = SMALL_ATOMS / 1024;
-__global__ __launch_bounds__(1024) void k_bin_small(BatchDev b, double *F) {
+__global__ __launch_bounds__(1024) void k_bin_small(BatchDev b) {
   __shared__ int cnt[SMALL_CELLS + 1];
   __shared__ int fill[SMALL_CELLS];
   __shared__ int slot[SMALL_ATOMS];
-  __shared__ int cel[SMALL_ATOMS];
   __shared__ int wsum[16];
   const int tid = threadIdx.x;
   const int nat = b.natoms;
@@ -147,30 +153,22 @@ __global__ __launch_bounds__(1024) void k_bin_small(BatchDev b, double *F) {
     cnt[c] = 0;
     fill[c] = 0;
   }
-  {
-    int as[SMALL_PER_THREAD];
-    double xs[SMALL_PER_THREAD][3];
+  // cells and lattice coordinates of this thread's atoms
+  int cs[SMALL_PER_THREAD];
+  double ss[SMALL_PER_THREAD][3];
 #pragma unroll
-    for (int u = 0; u < SMALL_PER_THREAD; ++u) {
-      const int g = tid + u * 1024;
-      if (g < nat) {
-        as[u] = b.atom_struct[g];
-        xs[u][0] = b.cart[3 * (size_t)g];
-        xs[u][1] = b.cart[3 * (size_t)g + 1];
-        xs[u][2] = b.cart[3 * (size_t)g + 2];
-      }
-    }
-    __syncthreads();  // counters cleared
-#pragma unroll
-    for (int u = 0; u < SMALL_PER_THREAD; ++u) {
-      const int g = tid + u * 1024;
-      if (g < nat) {
-        const int c = frac_cell_at(b, g, b.st + as[u], xs[u], F);
-        cel[g] = c;
-        atomicAdd(&cnt[c], 1);
-      }
+  for (int u = 0; u < SMALL_PER_THREAD; ++u) {
+    const int g = tid + u * 1024;
+    cs[u] = 0;
+    if (g < nat) {
+      cs[u] = b.cellof[g];
+      for (int k = 0; k < 3; ++k) ss[u][k] = b.s[3 * (size_t)g + k];
     }
   }
+  __syncthreads();  // counters cleared
+#pragma unroll
+  for (int u = 0; u < SMALL_PER_THREAD; ++u)
+    if (tid + u * 1024 < nat) atomicAdd(&cnt[cs[u]], 1);
   __syncthreads();
   // exclusive scan of cnt -> cell_start (global) and cnt (LDS, in place)
   int carry = 0;
@@ -205,44 +203,30 @@ __global__ __launch_bounds__(1024) void k_bin_small(BatchDev b, double *F) {
   }
   __syncthreads();
   for (int c = tid; c < b.ncells; c += blockDim.x) write_cell_work(b, c, cnt[c], cnt[c + 1], 0, 1);
-  for (int g = tid; g < nat; g += blockDim.x) {
-    const int c = cel[g];
-    slot[cnt[c] + atomicAdd(&fill[c], 1)] = g;
-  }
-  __syncthreads();
-  // rank within the cell, then the lattice coordinates into that order
-  int vs[SMALL_PER_THREAD], qs[SMALL_PER_THREAD];
 #pragma unroll
   for (int u = 0; u < SMALL_PER_THREAD; ++u) {
-    const int p = tid + u * 1024;
-    vs[u] = -1;
-    if (p < nat) {
-      const int v = slot[p];
-      const int c = cel[v];
+    const int g = tid + u * 1024;
+    if (g < nat) slot[cnt[cs[u]] + atomicAdd(&fill[cs[u]], 1)] = g;
+  }
+  __syncthreads();
+  // rank of each of this thread's atoms within its cell, then the atom and
+  // its lattice coordinates into that place
+#pragma unroll
+  for (int u = 0; u < SMALL_PER_THREAD; ++u) {
+    const int g = tid + u * 1024;
+    if (g < nat) {
+      const int c = cs[u];
       const int start = cnt[c], n = cnt[c + 1] - start;
       // eight independent counts: the LDS reads of a cell issue together
       int r8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       int k = 0;
       for (; k + 8 <= n; k += 8) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) r8[e] += slot[start + k + e] > v;
+        for (int e = 0; e < 8; ++e) r8[e] += slot[start + k + e] > g;
       }
-      for (; k < n; ++k) r8[0] += slot[start + k] > v;
-      const int rank = ((r8[0] + r8[1]) + (r8[2] + r8[3])) + ((r8[4] + r8[5]) + (r8[6] + r8[7]));
-      vs[u] = v;
-      qs[u] = start + rank;
-    }
-  }
-  double ss[SMALL_PER_THREAD][3];
-#pragma unroll
-  for (int u = 0; u < SMALL_PER_THREAD; ++u)
-    if (vs[u] >= 0)
-      for (int k = 0; k < 3; ++k) ss[u][k] = b.s[3 * (size_t)vs[u] + k];
-#pragma unroll
-  for (int u = 0; u < SMALL_PER_THREAD; ++u) {
-    if (vs[u] >= 0) {
-      const int q = qs[u];
-      b.cell_atoms[q] = vs[u];
+      for (; k < n; ++k) r8[0] += slot[start + k] > g;
+      const int q = start + ((r8[0] + r8[1]) + (r8[2] + r8[3])) + ((r8[4] + r8[5]) + (r8[6] + r8[7]));
+      b.cell_atoms[q] = g;
       b.sorted_s[3 * (size_t)q] = ss[u][0];
       b.sorted_s[3 * (size_t)q + 1] = ss[u][1];
       b.sorted_s[3 * (size_t)q + 2] = ss[u][2];
@@ -535,17 +519,19 @@ constexpr int CS_QUEUE = 128;  // pending passing candidates per wavefront
 
 struct CellSearchLds {
   double *cx, *cy, *cz;  // [CS_CHUNK] candidate coordinates (Cartesian, A)
-  int *cj;               // [CS_CHUNK] candidate atom
+  float4 *cf;            // [CS_CHUNK] FP32 coordinates relative to the home cell centre; w: atom (bits)
   int *queue;            // [CS_WAVES * CS_QUEUE]
   int *ovf;              // [CS_WAVES] sink overflow word (the count decides)
   int *hdr;              // [2] groups, candidates
-  int *gk, *gn, *gpos;   // [kc] groups (runs of equal cell): first combination, images, first atom position
+  int *cc;               // [2*kc] the home cell's (cell, image rank) combinations
+  int *gk, *gn;          // [kc] groups (runs of equal cell): first combination, images
+  int *gpos;             // [2*kc] group: first atom position in cell order; [kc + k]: one past its last
   int *gitems;           // [kc+1] prefix of atoms x images
 };
 
 static inline size_t cell_search_lds_bytes(int kc) {
-  return (size_t)CS_CHUNK * (3 * sizeof(double) + sizeof(int)) +
-         sizeof(int) * ((size_t)CS_WAVES * (CS_QUEUE + 1) + 2 + 4 * (size_t)kc + 1);
+  return (size_t)CS_CHUNK * (3 * sizeof(double) + sizeof(float4)) +
+         sizeof(int) * ((size_t)CS_WAVES * (CS_QUEUE + 1) + 2 + 7 * (size_t)kc + 1);
 }
 
 __device__ inline CellSearchLds carve_cell_search(double *p, int kc) {
@@ -553,15 +539,16 @@ __device__ inline CellSearchLds carve_cell_search(double *p, int kc) {
   W.cx = p;
   W.cy = p + CS_CHUNK;
   W.cz = p + 2 * CS_CHUNK;
-  int *q = reinterpret_cast<int *>(p + 3 * CS_CHUNK);
-  W.cj = q;
-  W.queue = W.cj + CS_CHUNK;
+  W.cf = reinterpret_cast<float4 *>(p + 3 * CS_CHUNK);
+  int *q = reinterpret_cast<int *>(W.cf + CS_CHUNK);
+  W.queue = q;
   W.ovf = W.queue + CS_WAVES * CS_QUEUE;
   W.hdr = W.ovf + CS_WAVES;
-  W.gk = W.hdr + 2;
+  W.cc = W.hdr + 2;
+  W.gk = W.cc + 2 * kc;
   W.gn = W.gk + kc;
   W.gpos = W.gn + kc;
-  W.gitems = W.gpos + kc;
+  W.gitems = W.gpos + 2 * kc;
   return W;
 }
 
@@ -579,25 +566,50 @@ __global__ __launch_bounds__(CS_WAVES *WAVE) void k_search_cells(BatchDev b, NbD
   const int kc = b.combo_max;
   CellSearchLds W = carve_cell_search(cs_dyn, kc);
   const StructDev *S = b.st + b.cell_struct[h];
+  // this wavefront's central atom: its loads issue first
+  const int a = j * CS_WAVES + w;
+  const bool have = a < n;  // uniform per wavefront
+  const int g = have ? b.cell_atoms[start + a] : 0;
+  double si[3] = {0.0, 0.0, 0.0};
+  if (have) {
+    si[0] = b.s[3 * (size_t)g];
+    si[1] = b.s[3 * (size_t)g + 1];
+    si[2] = b.s[3 * (size_t)g + 2];
+  }
   double L[9];
 #pragma unroll
   for (int k = 0; k < 9; ++k) L[k] = S->L[k];
   const int *Tv = b.Tvec + 3 * S->T0;
   const double Rc2 = S->Rc2;
-  const int *combos = b.combos + 2 * (size_t)h * b.combo_cap;
+  const double ctr[3] = {b.cell_ctr[3 * h], b.cell_ctr[3 * h + 1], b.cell_ctr[3 * h + 2]};
+  // FP32 prefilter: candidates and the central atom lie within M = rc +
+  // 3 cell radii of the cell centre; rounding to FP32 and the FP32
+  // distance then err by less than 84 M^2 2^-24 < 1e-5 M^2, so a distance
+  // outside Rc2 -+ band decides like the FP64 test, and only candidates
+  // inside the band take the FP64 test
+  const double M = S->rc_list + 3.0 * S->cell_rad;
+  const double band = 1.0e-5 * M * M + 1.0e-6 * Rc2;
+  const float lo2 = (float)(Rc2 - band), hi2 = (float)(Rc2 + band);
+  // the home cell's combinations into LDS
   const int K = min(b.combo_count[h], kc);
-  // groups of the home cell's combinations and their candidate counts
+  {
+    const int *combos = b.combos + 2 * (size_t)h * b.combo_cap;
+    for (int q = tid; q < 2 * K; q += CS_WAVES * WAVE) W.cc[q] = combos[q];
+  }
+  __syncthreads();
+  // groups of the combinations and their candidate counts
   if (w == 0) {
     int ng = 0;
     for (int base = 0; base < K; base += WAVE) {
       const int q = base + lane;
-      const int c = q < K ? combos[2 * q] : -1;
-      const bool st = q < K && (q == 0 || combos[2 * (q - 1)] != c);
+      const int c = q < K ? W.cc[2 * q] : -1;
+      const bool st = q < K && (q == 0 || W.cc[2 * (q - 1)] != c);
       const unsigned long long m = wave_ballot(st);
       if (st) {
         const int k = ng + mask_prefix(m);
         W.gk[k] = q;
         W.gpos[k] = b.cell_start[c];
+        W.gpos[kc + k] = b.cell_start[c + 1];
       }
       ng += popc64(m);
     }
@@ -607,10 +619,9 @@ __global__ __launch_bounds__(CS_WAVES *WAVE) void k_search_cells(BatchDev b, NbD
       const int k = base + lane;
       int items = 0;
       if (k < ng) {
-        const int q0 = W.gk[k], q1 = k + 1 < ng ? W.gk[k + 1] : K;
-        const int c = combos[2 * q0];
-        W.gn[k] = q1 - q0;
-        items = (b.cell_start[c + 1] - b.cell_start[c]) * (q1 - q0);
+        const int q1 = k + 1 < ng ? W.gk[k + 1] : K;
+        W.gn[k] = q1 - W.gk[k];
+        items = (W.gpos[kc + k] - W.gpos[k]) * (q1 - W.gk[k]);
       }
       const int incl = wave_scan(items);
       if (k < ng) W.gitems[k] = carry + incl - items;
@@ -628,16 +639,9 @@ __global__ __launch_bounds__(CS_WAVES *WAVE) void k_search_cells(BatchDev b, NbD
   const int cap = LISTS ? o.nb_cap : nb.cap;
   int *count_out = LISTS ? o.nb_count : nb.count;
   int *queue = W.queue + w * CS_QUEUE;
-  const int a = j * CS_WAVES + w;
-  const bool have = a < n;  // uniform per wavefront
-  const int g = have ? b.cell_atoms[start + a] : 0;
-  double si[3] = {0.0, 0.0, 0.0}, coo1[3] = {0.0, 0.0, 0.0};
-  if (have) {
-    si[0] = b.s[3 * (size_t)g];
-    si[1] = b.s[3 * (size_t)g + 1];
-    si[2] = b.s[3 * (size_t)g + 2];
-    matvec_exact(L, si, coo1);
-  }
+  double coo1[3] = {0.0, 0.0, 0.0};
+  if (have) matvec_exact(L, si, coo1);
+  const float c1x = (float)(coo1[0] - ctr[0]), c1y = (float)(coo1[1] - ctr[1]), c1z = (float)(coo1[2] - ctr[2]);
 
   // entries of one central atom g among the cn staged candidates, appended
   // from slot count on; returns the new count
@@ -648,18 +652,28 @@ __global__ __launch_bounds__(CS_WAVES *WAVE) void k_search_cells(BatchDev b, NbD
       cart[0] = coo2[0] - coo1[0];
       cart[1] = coo2[1] - coo1[1];
       cart[2] = coo2[2] - coo1[2];
-      sink.put(slot, W.cj[q], coo2, cart, sqrt(dist2_exact(cart)));
+      sink.put(slot, __float_as_int(W.cf[q].w), coo2, cart, sqrt(dist2_exact(cart)));
     };
     int qn = 0;
     for (int base = 0; base < cn; base += WAVE) {
       const int q = base + lane;
-      bool pass = false;
+      bool pass = false, border = false;
       if (q < cn) {
-        double cart[3];
-        cart[0] = W.cx[q] - coo1[0];
-        cart[1] = W.cy[q] - coo1[1];
-        cart[2] = W.cz[q] - coo1[2];
-        pass = W.cj[q] != g && dist2_exact(cart) <= Rc2;
+        const float4 v = W.cf[q];
+        const float dx = v.x - c1x, dy = v.y - c1y, dz = v.z - c1z;
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        const bool other = __float_as_int(v.w) != g;
+        pass = other && d2 < lo2;
+        border = other && d2 >= lo2 && d2 <= hi2;
+      }
+      if (wave_ballot(border)) {
+        if (border) {
+          double cart[3];
+          cart[0] = W.cx[q] - coo1[0];
+          cart[1] = W.cy[q] - coo1[1];
+          cart[2] = W.cz[q] - coo1[2];
+          pass = dist2_exact(cart) <= Rc2;
+        }
       }
       const unsigned long long m = wave_ballot(pass);
       if (pass) queue[qn + mask_prefix(m)] = q;
@@ -702,7 +716,7 @@ __global__ __launch_bounds__(CS_WAVES *WAVE) void k_search_cells(BatchDev b, NbD
         const int ai = gn == 1 ? off : off / gn;
         const int tt = off - ai * gn;
         const int pos = W.gpos[lo] + ai;
-        const int t = combos[2 * (W.gk[lo] + tt) + 1];
+        const int t = W.cc[2 * (W.gk[lo] + tt) + 1];
         const double sj[3] = {b.sorted_s[3 * (size_t)pos], b.sorted_s[3 * (size_t)pos + 1],
                               b.sorted_s[3 * (size_t)pos + 2]};
         double coo2[3];
@@ -716,7 +730,8 @@ __global__ __launch_bounds__(CS_WAVES *WAVE) void k_search_cells(BatchDev b, NbD
         W.cx[i] = coo2[0];
         W.cy[i] = coo2[1];
         W.cz[i] = coo2[2];
-        W.cj[i] = b.cell_atoms[pos];
+        W.cf[i] = make_float4((float)(coo2[0] - ctr[0]), (float)(coo2[1] - ctr[1]), (float)(coo2[2] - ctr[2]),
+                              __int_as_float(b.cell_atoms[pos]));
       }
       __syncthreads();
       if (have) count = scan(sink, g, coo1, count, cn);
@@ -807,7 +822,8 @@ void launch_combos(const BatchDev &b, int *combo_count, int *combos, hipStream_t
 
 void launch_binning(const BatchDev &b, int *tmp, double *F, hipStream_t s) {
   if (b.natoms <= SMALL_ATOMS && b.ncells <= SMALL_CELLS) {
-    hipLaunchKernelGGL(k_bin_small, dim3(1), dim3(1024), 0, s, b, F);
+    hipLaunchKernelGGL(k_frac, dim3(nblk(b.natoms, 256)), dim3(256), 0, s, b, F);
+    hipLaunchKernelGGL(k_bin_small, dim3(1), dim3(1024), 0, s, b);
     return;
   }
   // cell_count is zero on entry (cleared at allocation and by k_cell_sort)

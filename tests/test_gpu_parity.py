@@ -92,6 +92,56 @@ def test_neighbor_lists_bit_exact(path, device):
         ann.release()
 
 
+def _cutoff_cases(rc_list):
+    """Structures whose pair distances crowd the list radius: a simple cubic
+    grid of spacing rc_list/2 (second shell on the radius) displaced by
+    10^U(-9,-2) A in random directions, and random atoms in a sheared cell."""
+    rng = np.random.default_rng(17)
+    a = rc_list / 2.0
+    g = np.array([(i, j, k) for i in range(4) for j in range(4) for k in range(4)], float) * a
+    d = rng.normal(size=g.shape)
+    d *= (10.0 ** rng.uniform(-9, -2, size=(len(g), 1))) / np.linalg.norm(d, axis=1, keepdims=True)
+    cubic = (np.eye(3) * 4 * a, g + d, rng.integers(1, 3, len(g)))
+    sheared_cell = np.array([[13.0, 0.0, 0.0], [3.5, 12.5, 0.0], [-2.0, 4.0, 12.0]])
+    frac = rng.uniform(size=(160, 3))
+    sheared = (sheared_cell, frac @ sheared_cell, rng.integers(1, 3, 160))
+    return [("cubic_on_radius", cubic), ("sheared_random", sheared)]
+
+
+def test_neighbor_lists_at_the_cutoff(device):
+    """The cell-major search decides most candidates in FP32 against a band
+    around the list radius and the rest with the reference's FP64 test:
+    lists bit-identical to the oracle's where distances crowd the radius."""
+    from aenet_amd.core import ANNPotentials
+    from aenet_amd.device import DeviceBatch
+    from oracle_lib import Oracle
+    names = ["A", "B"]
+    ann = ANNPotentials(pot_files("alloy", names))
+    o = Oracle(names, pot_files("alloy", names))
+    try:
+        rc_list = o.Rc_max + 1.0e-3
+        for name, (cell, pos, types) in _cutoff_cases(rc_list):
+            print("case", name, flush=True)
+            ref = o.neighbors(cell, True, types, pos, cap=200)
+            s = type("S", (), {})()
+            s.cell, s.pbc, s.positions, s.types = cell, True, pos, types
+            db = DeviceBatch([s], device=device)
+            nb = db.neighbors(200)
+            nnb = nb["nnb"].cpu().numpy()
+            near = np.abs(ref["dist"][ref["dist"] > 0] - rc_list) < 1e-3
+            print("entries %d, within 1e-3 A of the radius %d" % (ref["nnb"].sum(), near.sum()))
+            assert np.array_equal(nnb, ref["nnb"])
+            for i in range(len(nnb)):
+                n = nnb[i]
+                assert np.array_equal(nb["coo"][i, :n].cpu().numpy(), ref["coo"][i, :n])
+                assert np.array_equal(nb["dist"][i, :n].cpu().numpy(), ref["dist"][i, :n])
+                assert np.array_equal(nb["list"][i, :n].cpu().numpy(), ref["list"][i, :n])
+                assert np.array_equal(nb["type"][i, :n].cpu().numpy(), ref["type"][i, :n])
+            del db
+    finally:
+        ann.release()
+
+
 @pytest.mark.parametrize("case", ["ref_01_alloy", "ref_03_bp3_fcc"])
 def test_per_atom_api_matches_reference(case, device):
     """aenet_atomic_energy_and_forces on the reference's own neighbor lists
