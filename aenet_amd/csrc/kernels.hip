@@ -140,13 +140,17 @@ __global__ __launch_bounds__(256) void k_frac(BatchDev b, double *F) {
 // (descending atom index, as the reference's linked lists) with the lattice
 // coordinates copied into that order.  Each thread takes the atoms
 // tid + u*1024 (u < SMALL_PER_THREAD) and keeps their cells and lattice
-// coordinates in registers.
+// coordinates in registers.  In LDS each cell's atoms start at a multiple
+// of 4 slots, padded with -1, so the rank count reads 4 slots at a time.
 constexpr int SMALL_ATOMS = 8192, SMALL_CELLS = 3072, SMALL_PER_THREAD = SMALL_ATOMS / 1024;
+constexpr int SMALL_SLOTS = SMALL_ATOMS + 3 * SMALL_CELLS;
 __global__ __launch_bounds__(1024) void k_bin_small(BatchDev b) {
-  __shared__ int cnt[SMALL_CELLS + 1];
+  __shared__ int cnt[SMALL_CELLS + 1];  // counts, then first atom of each cell in cell order
+  __shared__ int pst[SMALL_CELLS + 1];  // first slot of each cell (multiple of 4)
   __shared__ int fill[SMALL_CELLS];
-  __shared__ int slot[SMALL_ATOMS];
+  __shared__ int4 slot4[SMALL_SLOTS / 4];
   __shared__ int wsum[16];
+  int *slot = reinterpret_cast<int *>(slot4);
   const int tid = threadIdx.x;
   const int nat = b.natoms;
   for (int c = tid; c < b.ncells; c += blockDim.x) {
@@ -170,11 +174,13 @@ __global__ __launch_bounds__(1024) void k_bin_small(BatchDev b) {
   for (int u = 0; u < SMALL_PER_THREAD; ++u)
     if (tid + u * 1024 < nat) atomicAdd(&cnt[cs[u]], 1);
   __syncthreads();
-  // exclusive scan of cnt -> cell_start (global) and cnt (LDS, in place)
+  // exclusive scan of (count, count rounded up to 4), packed in the low and
+  // high 16 bits (totals below 8192 and 17408) -> cell_start and the slots
   int carry = 0;
   for (int base = 0; base < b.ncells; base += 1024) {
     const int i = base + tid;
-    const int v = i < b.ncells ? cnt[i] : 0;
+    const int n = i < b.ncells ? cnt[i] : 0;
+    const int v = n | (((n + 3) & ~3) << 16);
     const int x = wave_scan(v);
     const int w = tid / WAVE;
     if (lane_id() == WAVE - 1) wsum[w] = x;
@@ -192,21 +198,26 @@ __global__ __launch_bounds__(1024) void k_bin_small(BatchDev b) {
     const int tot = wsum[15];
     __syncthreads();
     if (i < b.ncells) {
-      cnt[i] = off;
-      b.cell_start[i] = off;
+      cnt[i] = off & 0xffff;
+      pst[i] = off >> 16;
+      b.cell_start[i] = off & 0xffff;
     }
     carry += tot;
   }
   if (tid == 0) {
-    cnt[b.ncells] = carry;
-    b.cell_start[b.ncells] = carry;
+    cnt[b.ncells] = carry & 0xffff;
+    pst[b.ncells] = carry >> 16;
+    b.cell_start[b.ncells] = carry & 0xffff;
   }
   __syncthreads();
-  for (int c = tid; c < b.ncells; c += blockDim.x) write_cell_work(b, c, cnt[c], cnt[c + 1], 0, 1);
+  for (int c = tid; c < b.ncells; c += blockDim.x) {
+    write_cell_work(b, c, cnt[c], cnt[c + 1], 0, 1);
+    for (int p = pst[c] + cnt[c + 1] - cnt[c]; p < pst[c + 1]; ++p) slot[p] = -1;  // padding
+  }
 #pragma unroll
   for (int u = 0; u < SMALL_PER_THREAD; ++u) {
     const int g = tid + u * 1024;
-    if (g < nat) slot[cnt[cs[u]] + atomicAdd(&fill[cs[u]], 1)] = g;
+    if (g < nat) slot[pst[cs[u]] + atomicAdd(&fill[cs[u]], 1)] = g;
   }
   __syncthreads();
   // rank of each of this thread's atoms within its cell, then the atom and
@@ -216,16 +227,14 @@ __global__ __launch_bounds__(1024) void k_bin_small(BatchDev b) {
     const int g = tid + u * 1024;
     if (g < nat) {
       const int c = cs[u];
-      const int start = cnt[c], n = cnt[c + 1] - start;
-      // eight independent counts: the LDS reads of a cell issue together
-      int r8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      int k = 0;
-      for (; k + 8 <= n; k += 8) {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) r8[e] += slot[start + k + e] > g;
+      const int p0 = pst[c] >> 2, p1 = pst[c + 1] >> 2;
+      int r0 = 0, r1 = 0;
+      for (int p = p0; p < p1; ++p) {
+        const int4 e = slot4[p];
+        r0 += (e.x > g) + (e.y > g);
+        r1 += (e.z > g) + (e.w > g);
       }
-      for (; k < n; ++k) r8[0] += slot[start + k] > g;
-      const int q = start + ((r8[0] + r8[1]) + (r8[2] + r8[3])) + ((r8[4] + r8[5]) + (r8[6] + r8[7]));
+      const int q = cnt[c] + r0 + r1;
       b.cell_atoms[q] = g;
       b.sorted_s[3 * (size_t)q] = ss[u][0];
       b.sorted_s[3 * (size_t)q + 1] = ss[u][1];
