@@ -339,11 +339,9 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
   const int lane = lane_id();
   const int wave = threadIdx.x / WAVE;
   const int nw = P.waves;
-  const float *Wsrc = md.W;
   if (P.w_in_lds) {
     for (int k = threadIdx.x; k < P.wsize; k += 64 * nw) lds[k] = md.W[k];
     __syncthreads();
-    Wsrc = lds;
   }
   const int wsz = P.w_in_lds ? (P.wsize + 3) / 4 * 4 : 0;
   BPWave S = bp_carve(lds + wsz + wave * bp_wave_words(P), P);
@@ -506,7 +504,8 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
     if (o.desc_only) continue;
 
     // ------------------------------------------------------------ 4
-    const float yout = network(md, sp, Wsrc, S.net, FORCES);
+    const float yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES)
+                                  : network(md, sp, (global_weights)md.W, S.net, FORCES);
     if (lane == 0) {
       if (MODE == 0)
         o.E_atom[g] = atom_energy(sp, yout);

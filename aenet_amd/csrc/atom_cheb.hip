@@ -89,11 +89,9 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
   const int wave = threadIdx.x / WAVE;
   const int nw = P.waves;
   // weights of every species, once per workgroup
-  const float *Wsrc = md.W;
   if (P.w_in_lds) {
     for (int k = threadIdx.x; k < P.wsize; k += 64 * nw) lds[k] = md.W[k];
     __syncthreads();
-    Wsrc = lds;
   }
   const int wsz = P.w_in_lds ? (P.wsize + 3) / 4 * 4 : 0;
   WaveLds S = carve(lds + wsz + wave * wave_lds_words(P), P);
@@ -253,7 +251,8 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
     if (o.desc_only) continue;
 
     // ------------------------------------------------------------ 4
-    const float yout = network(md, sp, Wsrc, S.net, FORCES);
+    const float yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES)
+                                  : network(md, sp, (global_weights)md.W, S.net, FORCES);
     if (lane == 0) {
       if (MODE == 0)
         o.E_atom[g] = atom_energy(sp, yout);

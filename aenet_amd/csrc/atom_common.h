@@ -151,19 +151,25 @@ __device__ inline NetLds carve_net(float *p, int nsf, int nl, int hs) {
 // layer stride for a model whose widest non-input layer has hmax nodes
 __host__ __device__ inline int net_stride(int hmax) { return hmax <= 32 ? 32 : AENET_AMD_MAX_HIDDEN; }
 
+// Weight pointers with their address space stated: a plain pointer that
+// may point to LDS or to global memory is read with FLAT loads, each waited
+// for on both memory counters; these are read with DS or global loads.
+typedef const __attribute__((address_space(3))) float *lds_weights;
+typedef const __attribute__((address_space(1))) float *global_weights;
+
 // Feed-forward network of one species on the normalized descriptor in
 // N.gp: ff_eval (src/ext/feedforward.f90:586-655), lanes = nodes of a
 // layer.  With grad, also ff_deriv (:661-757) and the chain through the
 // normalization and the energy scale: N.coef[f] = dE_i/dG_f.  Returns the
-// network output y (all lanes).
-__device__ inline float network(const ModelDev &md, const SpeciesDev &sp, const float *W, const NetLds &N,
-                                bool grad) {
+// network output y (all lanes).  W is lds_weights or global_weights.
+template <class WPtr>
+__device__ inline float network(const ModelDev &md, const SpeciesDev &sp, WPtr W, const NetLds &N, bool grad) {
   const int lane = lane_id();
   const int nl = sp.nlayers;
   const float *in = N.gp;
   for (int l = 0; l + 1 < nl; ++l) {
     const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
-    const float *Wl = W + sp.w_off[l];
+    const WPtr Wl = W + sp.w_off[l];
     if (lane < n2) {
       float z0 = Wl[n1 * n2 + lane], z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;  // bias column
       int c = 0;
@@ -190,9 +196,9 @@ __device__ inline float network(const ModelDev &md, const SpeciesDev &sp, const 
   wave_sync();
   for (int l = nl - 2; l >= 0; --l) {
     const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
-    const float *Wl = W + sp.w_off[l];
+    const WPtr Wl = W + sp.w_off[l];
     for (int c = lane; c < n1; c += WAVE) {
-      const float *row = Wl + c * n2;
+      const WPtr row = Wl + c * n2;
       float s0 = 0.0f, s1 = 0.0f;
       int hh = 0;
       for (; hh + 1 < n2; hh += 2) {
