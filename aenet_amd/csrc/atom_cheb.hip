@@ -36,6 +36,7 @@ constexpr int AUXN = 64;  // angular neighbors whose per-neighbor terms are tabu
 // Per-wavefront LDS region, laid out after the weights in dynamic shared
 // memory; cap = neighbor capacity.
 struct WaveLds {
+  SpeciesDev *spc;  // the central atom's species record
   float4 *aux; // [AUXN] per angular neighbor: 1/r, fc, dfc/dr, typespin
   float *R;    // [3*cap]
   float *r;    // [cap]
@@ -61,12 +62,14 @@ struct ChebLaunch {
 };
 
 __host__ __device__ inline int wave_lds_words(const ChebLaunch &P) {
-  return (4 * AUXN + 10 * P.cap + net_lds_words(P.nsf, P.nl, P.hs) + 3) / 4 * 4;
+  return (SPECIES_WORDS + 4 * AUXN + 10 * P.cap + net_lds_words(P.nsf, P.nl, P.hs) + 3) / 4 * 4;
 }
 
 __device__ inline WaveLds carve(float *base, const ChebLaunch &P) {
   const int cap = P.cap;
   WaveLds w;
+  w.spc = reinterpret_cast<SpeciesDev *>(base);
+  base += SPECIES_WORDS;
   w.aux = reinterpret_cast<float4 *>(base);
   w.R = base + 4 * AUXN;
   w.r = w.R + 3 * cap;
@@ -144,7 +147,7 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
       continue;
     }
     wave_sync();
-    const SpeciesDev &sp = md.species[ti - 1];
+    const SpeciesDev &sp = stage_species(md, ti - 1, S.spc);
     const int rN = sp.r_N, aN = sp.a_N, nsf = sp.nsf;
     const bool multi = sp.multi != 0;
     const float rRc = sp.r_Rc, aRc = sp.a_Rc;

@@ -151,6 +151,22 @@ __device__ inline NetLds carve_net(float *p, int nsf, int nl, int hs) {
 // layer stride for a model whose widest non-input layer has hmax nodes
 __host__ __device__ inline int net_stride(int hmax) { return hmax <= 32 ? 32 : AENET_AMD_MAX_HIDDEN; }
 
+// A species record copied into a wavefront's LDS once per central atom:
+// its fields (layer sizes, offsets, typespin table, ...) are then LDS reads.
+// Read through the global pointer they are vector loads (the compiler does
+// not take the table as read-only), each a memory round trip on the atom's
+// critical path.
+constexpr int SPECIES_WORDS = (int)((sizeof(SpeciesDev) + 15) / 16 * 4);
+
+__device__ inline const SpeciesDev &stage_species(const ModelDev &md, int t, SpeciesDev *dst) {
+  static_assert(sizeof(SpeciesDev) % 4 == 0, "whole words");
+  const int *src = reinterpret_cast<const int *>(md.species + t);
+  int *d = reinterpret_cast<int *>(dst);
+  for (int w = lane_id(); w < (int)(sizeof(SpeciesDev) / 4); w += WAVE) d[w] = src[w];
+  wave_sync();
+  return *dst;
+}
+
 // Weight pointers with their address space stated: a plain pointer that
 // may point to LDS or to global memory is read with FLAT loads, each waited
 // for on both memory counters; these are read with DS or global loads.
