@@ -245,8 +245,10 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
       }
       if (f >= 0) {
         // ------------------------------------------------------ 3
-        double Gs = md.norm_scale[sp.norm_off + f] * (G - md.norm_avg[sp.norm_off + f]);
+        const double sc = md.norm_scale[sp.norm_off + f];
+        double Gs = sc * (G - md.norm_avg[sp.norm_off + f]);
         S.net.gp[f] = (float)Gs;
+        S.net.coef[f] = (float)sc;  // for the network's dE/dG
         if (o.G) o.G[(size_t)g * o.G_stride + f] = o.G_scaled ? Gs : G;
       }
     }
@@ -254,8 +256,8 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
     if (o.desc_only) continue;
 
     // ------------------------------------------------------------ 4
-    const float yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES)
-                                  : network(md, sp, (global_weights)md.W, S.net, FORCES);
+    const float yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES, true)
+                                  : network(md, sp, (global_weights)md.W, S.net, FORCES, true);
     if (lane == 0) {
       if (MODE == 0)
         o.E_atom[g] = atom_energy(sp, yout);

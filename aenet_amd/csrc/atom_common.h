@@ -177,9 +177,13 @@ typedef const __attribute__((address_space(1))) float *global_weights;
 // N.gp: ff_eval (src/ext/feedforward.f90:586-655), lanes = nodes of a
 // layer.  With grad, also ff_deriv (:661-757) and the chain through the
 // normalization and the energy scale: N.coef[f] = dE_i/dG_f.  Returns the
-// network output y (all lanes).  W is lds_weights or global_weights.
+// network output y (all lanes).  W is lds_weights or global_weights.  With
+// scale_in_coef, N.coef[f] holds the normalization scale of feature f on
+// entry (left there by the caller's normalization) instead of it being read
+// from global memory again.
 template <class WPtr>
-__device__ inline float network(const ModelDev &md, const SpeciesDev &sp, WPtr W, const NetLds &N, bool grad) {
+__device__ inline float network(const ModelDev &md, const SpeciesDev &sp, WPtr W, const NetLds &N, bool grad,
+                                bool scale_in_coef = false) {
   const int lane = lane_id();
   const int nl = sp.nlayers;
   const float *in = N.gp;
@@ -231,7 +235,8 @@ __device__ inline float network(const ModelDev &md, const SpeciesDev &sp, WPtr W
     dnext = t;
   }
   const float Es = (float)sp.E_scale;
-  for (int f = lane; f < sp.nsf; f += WAVE) N.coef[f] = Es * dcur[f] * (float)md.norm_scale[sp.norm_off + f];
+  for (int f = lane; f < sp.nsf; f += WAVE)
+    N.coef[f] = Es * dcur[f] * (scale_in_coef ? N.coef[f] : (float)md.norm_scale[sp.norm_off + f]);
   wave_sync();
   return yout;
 }
