@@ -98,6 +98,7 @@ struct AtomSizes {
   // launch shape chosen by measurement (0 / -1: the occupancy model's)
   int tune_waves;    // wavefronts per workgroup
   int tune_staged;   // network weights staged in LDS (0 / 1)
+  int nnorm;         // normalization entries of all species (staged with the weights)
 };
 
 // network training (train.hip): per-atom normalized descriptors in, atomic

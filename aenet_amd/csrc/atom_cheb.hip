@@ -59,7 +59,12 @@ struct ChebLaunch {
   const int *active;   // optional subset of central atoms (mode 0)
   const int *nactive;
   int tune_waves, tune_staged;  // host side: measured launch shape
+  int nnorm;     // normalization entries (averages and scales staged after the weights)
 };
+
+// words staged once per workgroup: the weights (rounded to 4 words), then
+// the normalization averages and scales (FP64)
+__host__ __device__ inline int staged_words(const ChebLaunch &P) { return (P.wsize + 3) / 4 * 4 + 4 * P.nnorm; }
 
 __host__ __device__ inline int wave_lds_words(const ChebLaunch &P) {
   return (SPECIES_WORDS + 4 * AUXN + 10 * P.cap + net_lds_words(P.nsf, P.nl, P.hs) + 3) / 4 * 4;
@@ -91,12 +96,18 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
   const int lane = lane_id();
   const int wave = threadIdx.x / WAVE;
   const int nw = P.waves;
-  // weights of every species, once per workgroup
+  // weights and normalization of every species, once per workgroup
+  const int wW = (P.wsize + 3) / 4 * 4;
+  double *lnorm = reinterpret_cast<double *>(lds + wW);  // [2*nnorm]: averages, scales
   if (P.w_in_lds) {
     for (int k = threadIdx.x; k < P.wsize; k += 64 * nw) lds[k] = md.W[k];
+    for (int k = threadIdx.x; k < P.nnorm; k += 64 * nw) {
+      lnorm[k] = md.norm_avg[k];
+      lnorm[P.nnorm + k] = md.norm_scale[k];
+    }
     __syncthreads();
   }
-  const int wsz = P.w_in_lds ? (P.wsize + 3) / 4 * 4 : 0;
+  const int wsz = P.w_in_lds ? staged_words(P) : 0;
   WaveLds S = carve(lds + wsz + wave * wave_lds_words(P), P);
 
   const int count = P.active ? *P.nactive : P.natoms;
@@ -245,8 +256,11 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
       }
       if (f >= 0) {
         // ------------------------------------------------------ 3
-        const double sc = md.norm_scale[sp.norm_off + f];
-        double Gs = sc * (G - md.norm_avg[sp.norm_off + f]);
+        // (staged copies read through LDS-typed pointers: DS loads)
+        typedef const __attribute__((address_space(3))) double *lds_f64;
+        const double sc = P.w_in_lds ? ((lds_f64)lnorm)[P.nnorm + sp.norm_off + f] : md.norm_scale[sp.norm_off + f];
+        const double av = P.w_in_lds ? ((lds_f64)lnorm)[sp.norm_off + f] : md.norm_avg[sp.norm_off + f];
+        double Gs = sc * (G - av);
         S.net.gp[f] = (float)Gs;
         S.net.coef[f] = (float)sc;  // for the network's dE/dG
         if (o.G) o.G[(size_t)g * o.G_stride + f] = o.G_scaled ? Gs : G;
@@ -419,7 +433,7 @@ template <int RN, int AN, int MODE, bool F>
 static void launch_one(const ModelDev &md, const BatchDev &b, const NbDev &nb, const OutDev &o, const EnvDev &e,
                        ChebLaunch P, hipStream_t stream) {
   PersistentShape ps;
-  const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
+  const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)staged_words(P) : 0;
   if (!persistent_shape(wbytes, 4 * (size_t)wave_lds_words(P), P.natoms, 4, ps, CHEB_MAX_WAVES, P.tune_waves,
                         P.tune_staged)) {
     // cannot happen within the capacity limits; refuse rather than overrun LDS
@@ -462,13 +476,14 @@ static ChebLaunch cheb_launch(int mode, const AtomSizes &z) {
   P.nactive = mode == 0 ? z.nactive : nullptr;
   P.tune_waves = z.tune_waves;
   P.tune_staged = z.tune_staged;
+  P.nnorm = z.nnorm;
   return P;
 }
 
 bool atom_cheb_shape_ok(const AtomSizes &z) {
   const ChebLaunch P = cheb_launch(0, z);
   PersistentShape ps;
-  const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)((P.wsize + 3) / 4 * 4) : 0;
+  const size_t wbytes = P.wsize <= 16384 ? 4 * (size_t)staged_words(P) : 0;
   return persistent_shape(wbytes, 4 * (size_t)wave_lds_words(P), P.natoms, 4, ps, CHEB_MAX_WAVES, P.tune_waves,
                           P.tune_staged);
 }

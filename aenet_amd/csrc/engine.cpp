@@ -17,6 +17,7 @@ int DeviceModel::upload(const PotSet &ps, hipStream_t s) {
   max_aN = ps.max_aN;
   nsf_max = ps.nsf_max;
   wsize = (int)ps.W.size();
+  nnorm = (int)ps.norm_avg.size();
   rc_max = ps.Rc_max;
   nl_max = 0;
   for (const SpeciesDev &sd : ps.species) {

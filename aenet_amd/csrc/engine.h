@@ -51,10 +51,10 @@ struct DeviceModel {
   DBuf<double> avg, scale;
   DBuf<BPDev> bp;
   DBuf<BPFunc> bpf;
-  int ntypes = 0, desc = 0, max_rN = 0, max_aN = 0, nsf_max = 0, wsize = 0, nl_max = 0, hmax = 1;
+  int ntypes = 0, desc = 0, max_rN = 0, max_aN = 0, nsf_max = 0, wsize = 0, nl_max = 0, hmax = 1, nnorm = 0;
   double rc_max = 0.0;
   AtomSizes sizes(int natoms, int cap, const int *active = nullptr, const int *nactive = nullptr) const {
-    return AtomSizes{natoms, cap, nsf_max, wsize, nl_max, hmax, active, nactive, 0, -1};
+    return AtomSizes{natoms, cap, nsf_max, wsize, nl_max, hmax, active, nactive, 0, -1, nnorm};
   }
   int upload(const PotSet &ps, hipStream_t s);
   ModelDev view() const;
