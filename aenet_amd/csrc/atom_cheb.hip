@@ -121,15 +121,34 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
       n = min(nbl.count[g], P.cap);
       const float4 *R4 = reinterpret_cast<const float4 *>(nbl.R) + (size_t)g * nbl.cap;
       const int *J = nbl.j + (size_t)g * nbl.cap;
-      for (int q = lane; q < n; q += WAVE) {
-        float4 v = R4[q];
-        int j = J[q];
-        S.R[3 * q] = v.x;
-        S.R[3 * q + 1] = v.y;
-        S.R[3 * q + 2] = v.z;
-        S.r[q] = v.w;
-        S.j[q] = j;
-        S.t[q] = b.types[j];
+      // every slot up to the capacity is read (all in bounds; those beyond
+      // the count go unused), so these loads do not wait for the count; the
+      // type gather, indexed by the slots, does
+      for (int q0 = 0; q0 < P.cap; q0 += 2 * WAVE) {
+        float4 v[2];
+        int j[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int q = q0 + u * WAVE + lane;
+          v[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+          j[u] = 0;
+          if (q < P.cap) {
+            v[u] = R4[q];
+            j[u] = J[q];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          const int q = q0 + u * WAVE + lane;
+          if (q < n) {
+            S.R[3 * q] = v[u].x;
+            S.R[3 * q + 1] = v[u].y;
+            S.R[3 * q + 2] = v[u].z;
+            S.r[q] = v[u].w;
+            S.j[q] = j[u];
+            S.t[q] = b.types[j[u]];
+          }
+        }
       }
     } else {
       ti = env.env_type[g];
