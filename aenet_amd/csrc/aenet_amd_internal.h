@@ -151,7 +151,7 @@ struct BatchDev {
   int combo_cap;
   int kept_cap;               // largest combination count of a home cell (search LDS size)
   int combo_max;              // largest combination count of a home cell, unclamped (cell search LDS)
-  int *cell_work;             // [work_cap] cell-major search items: home cell or -1
+  int *cell_work;             // [4*work_cap] cell-major search items: (home cell, first atom, atoms, structure) or -1
   int work_cap;               // natoms/8 + ncells + 1
   const int *Tvec;            // all lattice translations
   const int *Cvec;            // all stencil vectors

@@ -41,14 +41,19 @@ __global__ void k_scale_f64(double *p, double a, long n) {
 // of home cell c, in cell order.  The binning kernels list the items: item
 // (c, j) sits at slot start_c / CS_WAVES + c + j (start_c = first atom of c in
 // cell order), which is below the next cell's first slot; the slots between
-// are -1.  The slot count is at most natoms / CS_WAVES + ncells + 1.
+// are -1.  The slot count is at most natoms / CS_WAVES + ncells + 1.  A slot
+// holds (c, start_c, atoms of c, structure of c), so the search starts from
+// one load.
 constexpr int CS_WAVES = 8;
 
 __device__ inline void write_cell_work(const BatchDev &b, int c, int start, int next, int first, int stride) {
+  const int sid = b.cell_struct[c];
   const int base = start / CS_WAVES + c;
   const int end = c + 1 < b.ncells ? next / CS_WAVES + c + 1 : b.work_cap;
   const int items = (next - start + CS_WAVES - 1) / CS_WAVES;
-  for (int k = base + first; k < end; k += stride) b.cell_work[k] = k - base < items ? c : -1;
+  for (int k = base + first; k < end; k += stride)
+    reinterpret_cast<int4 *>(b.cell_work)[k] =
+        k - base < items ? make_int4(c, start, next - start, sid) : make_int4(-1, 0, 0, 0);
 }
 
 // ---------------------------------------------------------------------
@@ -567,14 +572,15 @@ __device__ inline CellSearchLds carve_cell_search(double *p, int kc) {
 template <bool LISTS>
 __global__ __launch_bounds__(CS_WAVES *WAVE) void k_search_cells(BatchDev b, NbDev nb, OutDev o) {
   extern __shared__ double cs_dyn[];
-  const int h = b.cell_work[blockIdx.x];
+  const int4 item = reinterpret_cast<const int4 *>(b.cell_work)[blockIdx.x];
+  const int h = item.x;
   if (h < 0) return;  // empty slot (uniform)
   const int tid = threadIdx.x, lane = lane_id(), w = tid / WAVE;
-  const int start = b.cell_start[h], n = b.cell_start[h + 1] - start;
+  const int start = item.y, n = item.z;
   const int j = (int)blockIdx.x - (start / CS_WAVES + h);
   const int kc = b.combo_max;
   CellSearchLds W = carve_cell_search(cs_dyn, kc);
-  const StructDev *S = b.st + b.cell_struct[h];
+  const StructDev *S = b.st + item.w;
   // this wavefront's central atom: its loads issue first
   const int a = j * CS_WAVES + w;
   const bool have = a < n;  // uniform per wavefront
