@@ -190,20 +190,32 @@ __device__ inline float network(const ModelDev &md, const SpeciesDev &sp, WPtr W
   for (int l = 0; l + 1 < nl; ++l) {
     const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
     const WPtr Wl = W + sp.w_off[l];
-    if (lane < n2) {
-      float z0 = Wl[n1 * n2 + lane], z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;  // bias column
-      int c = 0;
-      for (; c + 3 < n1; c += 4) {
-        z0 += Wl[c * n2 + lane] * in[c];
-        z1 += Wl[(c + 1) * n2 + lane] * in[c + 1];
-        z2 += Wl[(c + 2) * n2 + lane] * in[c + 2];
-        z3 += Wl[(c + 3) * n2 + lane] * in[c + 3];
+    // lanes = (node, part): with NP the node count rounded up to a power of
+    // two (n2 <= 64), the 64 / NP parts share the inputs and a butterfly over
+    // the part bits of the lane sums them
+    int NP = 1;
+    while (NP < n2) NP <<= 1;
+    const int node = lane & (NP - 1), part = lane / NP;
+    const int per = (n1 * NP + WAVE - 1) / WAVE;
+    const int c0 = min(n1, part * per), c1 = min(n1, c0 + per);
+    float z0 = 0.0f, z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;
+    if (node < n2) {
+      int c = c0;
+      for (; c + 3 < c1; c += 4) {
+        z0 += Wl[c * n2 + node] * in[c];
+        z1 += Wl[(c + 1) * n2 + node] * in[c + 1];
+        z2 += Wl[(c + 2) * n2 + node] * in[c + 2];
+        z3 += Wl[(c + 3) * n2 + node] * in[c + 3];
       }
-      for (; c < n1; ++c) z0 += Wl[c * n2 + lane] * in[c];
+      for (; c < c1; ++c) z0 += Wl[c * n2 + node] * in[c];
+    }
+    float z = (z0 + z1) + (z2 + z3);
+    for (int o = WAVE / 2; o >= NP; o >>= 1) z += __shfl_xor(z, o, WAVE);
+    if (part == 0 && node < n2) {
       float y, dy;
-      activate(sp.fa[l + 1], (z0 + z1) + (z2 + z3), y, dy);
-      N.act[(l + 1) * N.hs + lane] = y;
-      N.dact[(l + 1) * N.hs + lane] = dy;
+      activate(sp.fa[l + 1], z + Wl[n1 * n2 + node], y, dy);  // bias column
+      N.act[(l + 1) * N.hs + node] = y;
+      N.dact[(l + 1) * N.hs + node] = dy;
     }
     wave_sync();
     in = N.act + (l + 1) * N.hs;
@@ -217,17 +229,39 @@ __device__ inline float network(const ModelDev &md, const SpeciesDev &sp, WPtr W
   for (int l = nl - 2; l >= 0; --l) {
     const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
     const WPtr Wl = W + sp.w_off[l];
-    for (int c = lane; c < n1; c += WAVE) {
-      const WPtr row = Wl + c * n2;
+    if (n1 <= WAVE / 2) {
+      // lanes = (input c, part) as in the forward pass, parts share the nodes
+      int NP = 1;
+      while (NP < n1) NP <<= 1;
+      const int c = lane & (NP - 1), part = lane / NP;
+      const int per = (n2 * NP + WAVE - 1) / WAVE;
+      const int h0 = min(n2, part * per), h1 = min(n2, h0 + per);
       float s0 = 0.0f, s1 = 0.0f;
-      int hh = 0;
-      for (; hh + 1 < n2; hh += 2) {
-        s0 += row[hh] * dcur[hh];
-        s1 += row[hh + 1] * dcur[hh + 1];
+      if (c < n1) {
+        const WPtr row = Wl + c * n2;
+        int hh = h0;
+        for (; hh + 1 < h1; hh += 2) {
+          s0 += row[hh] * dcur[hh];
+          s1 += row[hh + 1] * dcur[hh + 1];
+        }
+        if (hh < h1) s0 += row[hh] * dcur[hh];
       }
-      if (hh < n2) s0 += row[hh] * dcur[hh];
       float s = s0 + s1;
-      dnext[c] = l > 0 ? s * N.dact[l * N.hs + c] : s;
+      for (int o = WAVE / 2; o >= NP; o >>= 1) s += __shfl_xor(s, o, WAVE);
+      if (part == 0 && c < n1) dnext[c] = l > 0 ? s * N.dact[l * N.hs + c] : s;
+    } else {
+      for (int c = lane; c < n1; c += WAVE) {
+        const WPtr row = Wl + c * n2;
+        float s0 = 0.0f, s1 = 0.0f;
+        int hh = 0;
+        for (; hh + 1 < n2; hh += 2) {
+          s0 += row[hh] * dcur[hh];
+          s1 += row[hh + 1] * dcur[hh + 1];
+        }
+        if (hh < n2) s0 += row[hh] * dcur[hh];
+        float s = s0 + s1;
+        dnext[c] = l > 0 ? s * N.dact[l * N.hs + c] : s;
+      }
     }
     wave_sync();
     float *t = dcur;
