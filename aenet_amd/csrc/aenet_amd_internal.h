@@ -128,6 +128,10 @@ struct ModelDev {
   int desc;
 };
 
+// wavefronts (central atoms of one home cell) per workgroup of the
+// cell-major neighbor search
+constexpr int CS_WAVES = 8;
+
 struct BatchDev {
   const StructDev *st;
   int nstruct;
@@ -152,7 +156,7 @@ struct BatchDev {
   int kept_cap;               // largest combination count of a home cell (search LDS size)
   int combo_max;              // largest combination count of a home cell, unclamped (cell search LDS)
   int *cell_work;             // [4*work_cap] cell-major search items: (home cell, first atom, atoms, structure) or -1
-  int work_cap;               // natoms/8 + ncells + 1
+  int work_cap;               // natoms/CS_WAVES + ncells + 1
   const int *Tvec;            // all lattice translations
   const int *Cvec;            // all stencil vectors
   const int *self_t;          // self-image ranks

@@ -130,7 +130,7 @@ int Batch::allocate(std::string &err) {
   bad |= combo_count_.alloc(p.ncells) != 0;
   bad |= combos_.alloc(2 * (size_t)p.ncells * combo_cap_) != 0;
   bad |= flags_.alloc(1) != 0;
-  work_cap_ = p.natoms / 8 + p.ncells + 1;
+  work_cap_ = p.natoms / CS_WAVES + p.ncells + 1;
   bad |= cell_work_.alloc(4 * (size_t)work_cap_) != 0;
   bad |= scratch_E_.alloc(p.natoms) != 0;
   if (bad) {

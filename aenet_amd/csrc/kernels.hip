@@ -43,8 +43,8 @@ __global__ void k_scale_f64(double *p, double a, long n) {
 // cell order), which is below the next cell's first slot; the slots between
 // are -1.  The slot count is at most natoms / CS_WAVES + ncells + 1.  A slot
 // holds (c, start_c, atoms of c, structure of c), so the search starts from
-// one load.
-constexpr int CS_WAVES = 8;
+// one load.  CS_WAVES is set in aenet_amd_internal.h (the host sizes the
+// list from it).
 
 __device__ inline void write_cell_work(const BatchDev &b, int c, int start, int next, int first, int stride) {
   const int sid = b.cell_struct[c];
@@ -234,6 +234,7 @@ __global__ __launch_bounds__(1024) void k_bin_small(BatchDev b) {
       const int c = cs[u];
       const int p0 = pst[c] >> 2, p1 = pst[c + 1] >> 2;
       int r0 = 0, r1 = 0;
+#pragma unroll 4
       for (int p = p0; p < p1; ++p) {
         const int4 e = slot4[p];
         r0 += (e.x > g) + (e.y > g);
