@@ -48,6 +48,22 @@ constexpr int MT = AENET_AMD_MAX_TYPES;
 
 }  // namespace
 
+// The symmetry-function tables do not change during a launch: read through
+// constant-address-space pointers they become scalar loads, and the
+// per-function branches (cutoff, kind, lambda) wave-uniform.
+typedef const __attribute__((address_space(4))) BPFunc *bp_funcs;
+typedef const __attribute__((address_space(4))) BPDev *bp_tables;
+
+__device__ __forceinline__ BPFunc load_fn(bp_funcs fn, int k) {
+  BPFunc f;
+  f.kind = fn[k].kind;
+  f.Rc = fn[k].Rc;
+  f.p1 = fn[k].p1;
+  f.p2 = fn[k].p2;
+  f.p3 = fn[k].p3;
+  return f;
+}
+
 struct BPWave {
   float *R;     // [3*cap] R_j - R_i
   float *r;     // [cap]
@@ -223,7 +239,7 @@ __device__ __forceinline__ void trip_cut(const Trip &T, float rc, TripCut &c) {
 // reduce-scatter over the lanes continues in FP64
 struct AngValueOp {
   const float *R, *r;
-  const BPFunc *fn;
+  bp_funcs fn;
   int nf;  // functions in this chunk
   float acc[ACH];
   __device__ __forceinline__ void operator()(int ja, int kb, bool valid) {
@@ -237,7 +253,7 @@ struct AngValueOp {
 #pragma unroll
     for (int k = 0; k < ACH; ++k) {
       if (k < nf) {
-        const BPFunc f = fn[k];
+        const BPFunc f = load_fn(fn, k);
         if (f.Rc != c.rc) trip_cut(T, f.Rc, c);
         const float lam = f.p1, zeta = f.p2, eta = f.p3;
         float arg, L;
@@ -265,7 +281,7 @@ struct AngValueOp {
 struct AngForceOp {
   const float *R, *r;
   float *fs;
-  const BPFunc *fn;
+  bp_funcs fn;
   const float *coef;  // dE/dG of the block's functions
   int nf;
   __device__ __forceinline__ void operator()(int ja, int kb, bool valid) {
@@ -290,7 +306,7 @@ struct AngForceOp {
       s_cos = s_E = s_eta = 0.0f;
     };
     for (int k = 0; k < nf; ++k) {
-      const BPFunc f = fn[k];
+      const BPFunc f = load_fn(fn, k);
       const bool g4 = f.kind == 4;
       if (f.Rc != c.rc || g4 != seg_g4) {
         if (c.rc > 0.0f) flush();
@@ -345,8 +361,9 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
   }
   const int wsz = P.w_in_lds ? (P.wsize + 3) / 4 * 4 : 0;
   BPWave S = bp_carve(lds + wsz + wave * bp_wave_words(P), P);
-  const BPDev &bp = *md.bp;
-  const int nt = bp.ntypes;
+  const bp_tables bpt = (bp_tables)md.bp;
+  const bp_funcs bpf = (bp_funcs)md.bpf;
+  const int nt = bpt->ntypes;
 
   const int count = P.active ? *P.nactive : P.natoms;
   for (int k = blockIdx.x * nw + wave; k < count; k += gridDim.x * nw) {
@@ -436,8 +453,8 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
     // radial blocks
     for (int t2 = 0; t2 < nt; ++t2) {
       const int key2 = row * MT + t2;
-      const int nf = bp.rad_n[key2], off = bp.rad_off[key2];
-      const BPFunc *fn = md.bpf + bp.rad_p0[key2];
+      const int nf = bpt->rad_n[key2], off = bpt->rad_off[key2];
+      const bp_funcs fn = bpf + bpt->rad_p0[key2];
       const int bs = S.bstart[t2], m = S.bstart[t2 + 1] - bs;
       for (int c0 = 0; c0 < nf; c0 += RCH) {
         float acc[RCH];
@@ -449,7 +466,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
 #pragma unroll
           for (int k = 0; k < RCH; ++k) {
             if (c0 + k < nf) {
-              const BPFunc f = fn[c0 + k];
+              const BPFunc f = load_fn(fn, c0 + k);
               if (f.Rc != rc) {
                 rc = f.Rc;
                 cutoff(r, rc, 1.0f / rc, fc, dfc);
@@ -474,12 +491,12 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
     for (int t2 = 0; t2 < nt; ++t2) {
       for (int t3 = t2; t3 < nt; ++t3) {
         const int key3 = (row * MT + t2) * MT + t3;
-        const int nf = bp.ang_n[key3], off = bp.ang_off[key3];
+        const int nf = bpt->ang_n[key3], off = bpt->ang_off[key3];
         if (nf == 0) continue;
-        const BPFunc *fn = md.bpf + bp.ang_p0[key3];
+        const bp_funcs fn = bpf + bpt->ang_p0[key3];
         const int b2 = S.bstart[t2], m2 = S.bstart[t2 + 1] - b2;
         const int b3 = S.bstart[t3], m3 = S.bstart[t3 + 1] - b3;
-        const float rc = bp.ang_rcmax[key3];
+        const float rc = bpt->ang_rcmax[key3];
         for (int c0 = 0; c0 < nf; c0 += ACH) {
           AngValueOp op;
           op.R = S.R;
@@ -488,7 +505,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
           op.nf = min(ACH, nf - c0);
 #pragma unroll
           for (int k = 0; k < ACH; ++k) op.acc[k] = 0.0f;
-          for_triplets(S, b2, m2, b3, m3, t2 == t3, bp.ang_g5[key3] != 0, rc * rc, op);
+          for_triplets(S, b2, m2, b3, m3, t2 == t3, bpt->ang_g5[key3] != 0, rc * rc, op);
           const double G = reduce_scatter<ACH>(op.acc);
           const int idx = lane >> 1;
           if ((lane & 1) == 0 && c0 + idx < nf) {
@@ -524,16 +541,16 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
     const float *coef = S.net.coef;
     for (int t2 = 0; t2 < nt; ++t2) {
       const int key2 = row * MT + t2;
-      const int nf = bp.rad_n[key2], off = bp.rad_off[key2];
+      const int nf = bpt->rad_n[key2], off = bpt->rad_off[key2];
       if (nf == 0) continue;
-      const BPFunc *fn = md.bpf + bp.rad_p0[key2];
+      const bp_funcs fn = bpf + bpt->rad_p0[key2];
       const int bs = S.bstart[t2], m = S.bstart[t2 + 1] - bs;
       for (int q = lane; q < m; q += WAVE) {
         const int nq = S.bidx[bs + q];
         const float r = S.r[nq];
         float rc = -1.0f, fc = 0.0f, dfc = 0.0f, s = 0.0f;
         for (int k = 0; k < nf; ++k) {
-          const BPFunc f = fn[k];
+          const BPFunc f = load_fn(fn, k);
           if (f.Rc != rc) {
             rc = f.Rc;
             cutoff(r, rc, 1.0f / rc, fc, dfc);
@@ -553,19 +570,19 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
     for (int t2 = 0; t2 < nt; ++t2) {
       for (int t3 = t2; t3 < nt; ++t3) {
         const int key3 = (row * MT + t2) * MT + t3;
-        const int nf = bp.ang_n[key3], off = bp.ang_off[key3];
+        const int nf = bpt->ang_n[key3], off = bpt->ang_off[key3];
         if (nf == 0) continue;
         const int b2 = S.bstart[t2], m2 = S.bstart[t2 + 1] - b2;
         const int b3 = S.bstart[t3], m3 = S.bstart[t3 + 1] - b3;
-        const float rc = bp.ang_rcmax[key3];
+        const float rc = bpt->ang_rcmax[key3];
         AngForceOp op;
         op.R = S.R;
         op.r = S.r;
         op.fs = S.fs;
-        op.fn = md.bpf + bp.ang_p0[key3];
+        op.fn = bpf + bpt->ang_p0[key3];
         op.coef = coef + off;
         op.nf = nf;
-        for_triplets(S, b2, m2, b3, m3, t2 == t3, bp.ang_g5[key3] != 0, rc * rc, op);
+        for_triplets(S, b2, m2, b3, m3, t2 == t3, bpt->ang_g5[key3] != 0, rc * rc, op);
       }
     }
     wave_sync();
