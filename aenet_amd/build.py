@@ -53,14 +53,20 @@ def build(verbose=True, jobs=8):
         o = os.path.join(obj_dir, src + ".o")
         cmds.append([CXX, *COMMON, *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
+    # a failed compile must stop the build: linking would pick up the
+    # object file an earlier build left behind
+    for o in objs:
+        if os.path.exists(o):
+            os.remove(o)
     procs = []
+    rc = []
     for c in cmds:
         if verbose:
             print(" ".join(c), flush=True)
         procs.append(subprocess.Popen(c))
         if len(procs) >= jobs:
-            procs.pop(0).wait()
-    rc = [p.wait() for p in procs]
+            rc.append(procs.pop(0).wait())
+    rc += [p.wait() for p in procs]
     if any(rc):
         raise RuntimeError("compilation failed")
     tmp = LIB + ".tmp"

@@ -54,6 +54,13 @@ constexpr int MT = AENET_AMD_MAX_TYPES;
 typedef const __attribute__((address_space(4))) BPFunc *bp_funcs;
 typedef const __attribute__((address_space(4))) BPDev *bp_tables;
 
+// 2^x and log2(x) as the hardware instructions (1 ulp).  The library forms
+// add scaling for subnormal inputs and results only: the log2 arguments here
+// are clamped at 1e-30, and a term below 2^-126 vanishes beside the sums it
+// joins.
+__device__ __forceinline__ float exp2_hw(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ __forceinline__ float log2_hw(float x) { return __builtin_amdgcn_logf(x); }
+
 __device__ __forceinline__ BPFunc load_fn(bp_funcs fn, int k) {
   BPFunc f;
   f.kind = fn[k].kind;
@@ -114,7 +121,7 @@ __device__ inline BPWave bp_carve(float *base, const BPLaunch &P) {
 __device__ __forceinline__ void radial_fn(const BPFunc &f, float r, float fc, float dfc, float &v, float &dv) {
   if (f.kind == 2) {
     const float a = r - f.p1;
-    const float e = exp2f(-f.p2 * LOG2E * a * a);
+    const float e = exp2_hw(-f.p2 * LOG2E * a * a);
     v = e * fc;
     dv = e * (dfc - 2.0f * f.p2 * a * fc);
   } else if (f.kind == 3) {
@@ -246,7 +253,7 @@ struct AngValueOp {
     if (!valid) return;
     const Trip T = make_trip(R, r, ja, kb);
     const float argp = 0.5f * (1.0f + T.cosv), argm = 0.5f * (1.0f - T.cosv);
-    const float Lp = log2f(fmaxf(argp, 1.0e-30f)), Lm = log2f(fmaxf(argm, 1.0e-30f));
+    const float Lp = log2_hw(fmaxf(argp, 1.0e-30f)), Lm = log2_hw(fmaxf(argm, 1.0e-30f));
     const float S2 = T.rj * T.rj + T.rk * T.rk, S3 = S2 + T.rjk * T.rjk;
     TripCut c;
     c.rc = -1.0f;
@@ -265,10 +272,10 @@ struct AngValueOp {
           L = Lm;
         } else {
           arg = 0.5f * (1.0f + lam * T.cosv);
-          L = log2f(fmaxf(arg, 1.0e-30f));
+          L = log2_hw(fmaxf(arg, 1.0e-30f));
         }
         const bool g4 = f.kind == 4;
-        const float P = 2.0f * exp2f((zeta - 1.0f) * L - eta * LOG2E * (g4 ? S3 : S2));
+        const float P = 2.0f * exp2_hw((zeta - 1.0f) * L - eta * LOG2E * (g4 ? S3 : S2));
         const float C = g4 ? c.cj * c.ck * c.cjk : c.cj * c.ck;
         acc[k] += P * arg * C;
       }
@@ -288,7 +295,7 @@ struct AngForceOp {
     if (!valid) return;
     const Trip T = make_trip(R, r, ja, kb);
     const float argp = 0.5f * (1.0f + T.cosv), argm = 0.5f * (1.0f - T.cosv);
-    const float Lp = log2f(fmaxf(argp, 1.0e-30f)), Lm = log2f(fmaxf(argm, 1.0e-30f));
+    const float Lp = log2_hw(fmaxf(argp, 1.0e-30f)), Lm = log2_hw(fmaxf(argm, 1.0e-30f));
     const float S2 = T.rj * T.rj + T.rk * T.rk, S3 = S2 + T.rjk * T.rjk;
     float dRj = 0.0f, dRk = 0.0f, dRjk = 0.0f, dcos = 0.0f;
     // sums over a segment of functions with the same kind and cutoff
@@ -323,9 +330,9 @@ struct AngForceOp {
         L = Lm;
       } else {
         arg = 0.5f * (1.0f + lam * T.cosv);
-        L = log2f(fmaxf(arg, 1.0e-30f));
+        L = log2_hw(fmaxf(arg, 1.0e-30f));
       }
-      const float P = 2.0f * exp2f((zeta - 1.0f) * L - eta * LOG2E * (g4 ? S3 : S2));
+      const float P = 2.0f * exp2_hw((zeta - 1.0f) * L - eta * LOG2E * (g4 ? S3 : S2));
       const float w = coef[k];
       const float wE = w * P * arg;
       s_cos += w * (0.5f * zeta * lam) * P;
