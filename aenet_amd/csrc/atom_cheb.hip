@@ -115,18 +115,17 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
     const int g = P.active ? P.active[k] : k;
     wave_sync();
     // ------------------------------------------------------------ 1
-    int ti, n;
+    int ti, n, spw;
     if (MODE == 0) {
+      // Two dependent memory rounds per atom: (type, count, list slots),
+      // which depend on g alone, then (species record, neighbor types).
+      // Every slot up to the capacity is read (all in bounds; those beyond
+      // the count go unused), so the list loads do not wait for the count.
       ti = b.types[g];
-      n = min(nbl.count[g], P.cap);
+      const int cnt = nbl.count[g];
       const float4 *R4 = reinterpret_cast<const float4 *>(nbl.R) + (size_t)g * nbl.cap;
       const int *J = nbl.j + (size_t)g * nbl.cap;
-      // every slot up to the capacity is read (all in bounds; those beyond
-      // the count go unused), so these loads do not wait for the count; the
-      // type gather, indexed by the slots, does
-      for (int q0 = 0; q0 < P.cap; q0 += 2 * WAVE) {
-        float4 v[2];
-        int j[2];
+      auto load_slots = [&](int q0, float4 (&v)[2], int (&j)[2]) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int q = q0 + u * WAVE + lane;
@@ -137,6 +136,17 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
             j[u] = J[q];
           }
         }
+      };
+      auto gather_types = [&](int q0, const int (&j)[2], int (&tj)[2]) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          // unconditional (slots past the count read the atom's own type),
+          // so both gathers issue back to back
+          const int jj = q0 + u * WAVE + lane < n ? j[u] : g;
+          tj[u] = b.types[jj];
+        }
+      };
+      auto put_slots = [&](int q0, const float4 (&v)[2], const int (&j)[2], const int (&tj)[2]) {
 #pragma unroll
         for (int u = 0; u < 2; ++u) {
           const int q = q0 + u * WAVE + lane;
@@ -146,12 +156,27 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
             S.R[3 * q + 2] = v[u].z;
             S.r[q] = v[u].w;
             S.j[q] = j[u];
-            S.t[q] = b.types[j[u]];
+            S.t[q] = tj[u];
           }
         }
+      };
+      float4 v[2];
+      int j[2], tj[2];
+      load_slots(0, v, j);
+      n = min(cnt, P.cap);
+      gather_types(0, j, tj);
+      // issued after the gathers: loads complete in order, so the stores
+      // below wait for the gathers without waiting for this one
+      spw = species_word(md, min(max(ti, 1), md.ntypes) - 1);  // clamped: a bad type is refused below
+      put_slots(0, v, j, tj);
+      for (int q0 = 2 * WAVE; q0 < P.cap; q0 += 2 * WAVE) {
+        load_slots(q0, v, j);
+        gather_types(q0, j, tj);
+        put_slots(q0, v, j, tj);
       }
     } else {
       ti = env.env_type[g];
+      spw = species_word(md, min(max(ti, 1), md.ntypes) - 1);
       int e0 = env.env_start[g];
       n = env.env_start[g + 1] - e0;
       if (n > P.cap) {
@@ -177,7 +202,7 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
       continue;
     }
     wave_sync();
-    const SpeciesDev &sp = stage_species(md, ti - 1, S.spc);
+    const SpeciesDev &sp = store_species(spw, S.spc);
     const int rN = sp.r_N, aN = sp.a_N, nsf = sp.nsf;
     const bool multi = sp.multi != 0;
     const float rRc = sp.r_Rc, aRc = sp.a_Rc;

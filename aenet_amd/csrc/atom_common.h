@@ -158,13 +158,25 @@ __host__ __device__ inline int net_stride(int hmax) { return hmax <= 32 ? 32 : A
 // critical path.
 constexpr int SPECIES_WORDS = (int)((sizeof(SpeciesDev) + 15) / 16 * 4);
 
-__device__ inline const SpeciesDev &stage_species(const ModelDev &md, int t, SpeciesDev *dst) {
-  static_assert(sizeof(SpeciesDev) % 4 == 0, "whole words");
-  const int *src = reinterpret_cast<const int *>(md.species + t);
-  int *d = reinterpret_cast<int *>(dst);
-  for (int w = lane_id(); w < (int)(sizeof(SpeciesDev) / 4); w += WAVE) d[w] = src[w];
+// The copy in two halves, one word of the record per lane: species_word's
+// load can issue as soon as the type is known, beside the neighbor-list
+// loads, and store_species writes the words to LDS once they are needed.
+static_assert(sizeof(SpeciesDev) % 4 == 0 && sizeof(SpeciesDev) / 4 <= WAVE, "one word per lane");
+
+__device__ __forceinline__ int species_word(const ModelDev &md, int t) {
+  const int w = lane_id();
+  return w < (int)(sizeof(SpeciesDev) / 4) ? reinterpret_cast<const int *>(md.species + t)[w] : 0;
+}
+
+__device__ __forceinline__ const SpeciesDev &store_species(int word, SpeciesDev *dst) {
+  const int w = lane_id();
+  if (w < (int)(sizeof(SpeciesDev) / 4)) reinterpret_cast<int *>(dst)[w] = word;
   wave_sync();
   return *dst;
+}
+
+__device__ inline const SpeciesDev &stage_species(const ModelDev &md, int t, SpeciesDev *dst) {
+  return store_species(species_word(md, t), dst);
 }
 
 // Weight pointers with their address space stated: a plain pointer that
