@@ -146,7 +146,11 @@ int Batch::allocate(std::string &err) {
                          d.L[6] * (d.L[1] * d.L[5] - d.L[2] * d.L[4]));
     if (si < occ_.size()) V *= occ_[si];
     double rho = V > 0.0 ? d.natoms / V : 0.0;
-    est = std::max(est, rho * 4.18879020478639 * d.rc_list * d.rc_list * d.rc_list + d.nself);
+    double e = rho * 4.18879020478639 * d.rc_list * d.rc_list * d.rc_list + d.nself;
+    // an isolated structure has no images: at most natoms - 1 neighbors
+    // (its bounding box overstates the density of a small cluster)
+    if (!d.pbc) e = std::min(e, (double)(d.natoms - 1));
+    est = std::max(est, e);
   }
   int cap = (int)(1.2 * est) + 16;
   cap = std::max(64, std::min(AENET_AMD_NB_CAP, (cap + 15) / 16 * 16));
