@@ -72,10 +72,11 @@ __device__ __forceinline__ BPFunc load_fn(bp_funcs fn, int k) {
 }
 
 struct BPWave {
+  float2 *ct;   // [cap] cutoff fc and dfc/dr of each neighbor at the angular table radius
   float *R;     // [3*cap] R_j - R_i
   float *r;     // [cap]
-  int *t;       // [cap] local type of the neighbor (0-based; -1: not in the descriptor)
-  int *j;       // [cap] neighbor ids (global atom or environment entry)
+  int *t;       // [cap] local type of the neighbor (0-based; -1: not in the descriptor); the
+                // first words of fs, used only before the gradients are cleared
   int *bidx;    // [cap] neighbor slots grouped by local type
   float *fs;    // [3*cap] per-neighbor gradients dE_i/dR_j
   int *queue;   // [QCAP] packed (j slot, k slot) of pending triplets
@@ -98,18 +99,18 @@ struct BPLaunch {
 };
 
 __host__ __device__ inline int bp_wave_words(const BPLaunch &P) {
-  return 10 * P.cap + QCAP + 16 + net_lds_words(P.nsf, P.nl, P.hs);
+  return (10 * P.cap + QCAP + 16 + net_lds_words(P.nsf, P.nl, P.hs) + 3) / 4 * 4;
 }
 
 __device__ inline BPWave bp_carve(float *base, const BPLaunch &P) {
   const int cap = P.cap;
   BPWave w;
-  w.R = base;
+  w.ct = reinterpret_cast<float2 *>(base);  // base is 16-byte aligned
+  w.R = base + 2 * cap;
   w.r = w.R + 3 * cap;
-  w.t = reinterpret_cast<int *>(w.r + cap);
-  w.j = w.t + cap;
-  w.bidx = w.j + cap;
+  w.bidx = reinterpret_cast<int *>(w.r + cap);
   w.fs = reinterpret_cast<float *>(w.bidx + cap);
+  w.t = reinterpret_cast<int *>(w.fs);
   w.queue = reinterpret_cast<int *>(w.fs + 3 * cap);
   w.bstart = w.queue + QCAP;
   w.net = carve_net(reinterpret_cast<float *>(w.bstart + 16), P.nsf, P.nl, P.hs);
@@ -233,11 +234,29 @@ struct TripCut {
   float cj, dcj, ck, dck, cjk, dcjk;
 };
 
-__device__ __forceinline__ void trip_cut(const Trip &T, float rc, TripCut &c) {
-  const float inv = 1.0f / rc;
+// per-neighbor cutoff table at one radius (the row's first angular cutoff)
+struct CutTab {
+  const float2 *ct;
+  float rc, inv;
+};
+
+// the legs j and k come from the table when rc is its radius (the same
+// cutoff() of the same r, computed once per neighbor instead of per triplet)
+__device__ __forceinline__ void trip_cut(const Trip &T, float rc, TripCut &c, const CutTab &tab, int ja, int kb) {
   c.rc = rc;
-  cutoff(T.rj, rc, inv, c.cj, c.dcj);
-  cutoff(T.rk, rc, inv, c.ck, c.dck);
+  float inv;
+  if (rc == tab.rc) {
+    inv = tab.inv;
+    const float2 a = tab.ct[ja], b2 = tab.ct[kb];
+    c.cj = a.x;
+    c.dcj = a.y;
+    c.ck = b2.x;
+    c.dck = b2.y;
+  } else {
+    inv = 1.0f / rc;
+    cutoff(T.rj, rc, inv, c.cj, c.dcj);
+    cutoff(T.rk, rc, inv, c.ck, c.dck);
+  }
   cutoff(T.rjk, rc, inv, c.cjk, c.dcjk);
 }
 
@@ -246,6 +265,7 @@ __device__ __forceinline__ void trip_cut(const Trip &T, float rc, TripCut &c) {
 // reduce-scatter over the lanes continues in FP64
 struct AngValueOp {
   const float *R, *r;
+  CutTab tab;
   bp_funcs fn;
   int nf;  // functions in this chunk
   float acc[ACH];
@@ -261,7 +281,7 @@ struct AngValueOp {
     for (int k = 0; k < ACH; ++k) {
       if (k < nf) {
         const BPFunc f = load_fn(fn, k);
-        if (f.Rc != c.rc) trip_cut(T, f.Rc, c);
+        if (f.Rc != c.rc) trip_cut(T, f.Rc, c, tab, ja, kb);
         const float lam = f.p1, zeta = f.p2, eta = f.p3;
         float arg, L;
         if (lam == 1.0f) {
@@ -287,6 +307,7 @@ struct AngValueOp {
 // function of the block, then the per-neighbor gradients
 struct AngForceOp {
   const float *R, *r;
+  CutTab tab;
   float *fs;
   bp_funcs fn;
   const float *coef;  // dE/dG of the block's functions
@@ -317,7 +338,7 @@ struct AngForceOp {
       const bool g4 = f.kind == 4;
       if (f.Rc != c.rc || g4 != seg_g4) {
         if (c.rc > 0.0f) flush();
-        if (f.Rc != c.rc) trip_cut(T, f.Rc, c);
+        if (f.Rc != c.rc) trip_cut(T, f.Rc, c, tab, ja, kb);
         seg_g4 = g4;
       }
       const float lam = f.p1, zeta = f.p2, eta = f.p3;
@@ -410,11 +431,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
         S.R[3 * q + 1] = v.y;
         S.R[3 * q + 2] = v.z;
         S.r[q] = v.w;
-        S.j[q] = j;
         S.t[q] = lt;
-        S.fs[3 * q] = 0.0f;
-        S.fs[3 * q + 1] = 0.0f;
-        S.fs[3 * q + 2] = 0.0f;
       }
     } else {
       const int e0 = env.env_start[g];
@@ -429,11 +446,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
         S.R[3 * q + 1] = (float)c[1];
         S.R[3 * q + 2] = (float)c[2];
         S.r[q] = (float)sqrt(dist2_exact(c));
-        S.j[q] = e0 + q;
         S.t[q] = lt;
-        S.fs[3 * q] = 0.0f;
-        S.fs[3 * q + 1] = 0.0f;
-        S.fs[3 * q + 2] = 0.0f;
       }
     }
     if (wave_ballot(bad_nb) != 0ull && lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_BAD_TYPE);
@@ -455,6 +468,28 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
     }
     wave_sync();
     const int row = sp.bp_row;
+    // the local types (the first words of fs) are done with: clear the
+    // gradients, and tabulate each neighbor's cutoff at the radius of the
+    // row's first angular function
+    CutTab tab;
+    tab.ct = S.ct;
+    tab.rc = -1.0f;
+    tab.inv = 0.0f;
+    for (int t2 = 0; t2 < nt && tab.rc < 0.0f; ++t2)
+      for (int t3 = t2; t3 < nt && tab.rc < 0.0f; ++t3) {
+        const int key3 = (row * MT + t2) * MT + t3;
+        if (bpt->ang_n[key3] > 0) tab.rc = bpf[bpt->ang_p0[key3]].Rc;
+      }
+    if (tab.rc > 0.0f) tab.inv = 1.0f / tab.rc;
+    for (int q = lane; q < n; q += WAVE) {
+      S.fs[3 * q] = 0.0f;
+      S.fs[3 * q + 1] = 0.0f;
+      S.fs[3 * q + 2] = 0.0f;
+      float fc = 0.0f, dfc = 0.0f;
+      if (tab.rc > 0.0f) cutoff(S.r[q], tab.rc, tab.inv, fc, dfc);
+      S.ct[q] = make_float2(fc, dfc);
+    }
+    wave_sync();
 
     // ------------------------------------------------------------ 2 + 3
     // radial blocks
@@ -508,6 +543,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
           AngValueOp op;
           op.R = S.R;
           op.r = S.r;
+          op.tab = tab;
           op.fn = fn + c0;
           op.nf = min(ACH, nf - c0);
 #pragma unroll
@@ -585,6 +621,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
         AngForceOp op;
         op.R = S.R;
         op.r = S.r;
+        op.tab = tab;
         op.fs = S.fs;
         op.fn = bpf + bpt->ang_p0[key3];
         op.coef = coef + off;
@@ -595,18 +632,20 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
     wave_sync();
     // F_j -= dE_i/dR_j ; F_i += sum_j dE_i/dR_j  (src/aenet.f90:471-477)
     double own[3] = {0.0, 0.0, 0.0};
+    const int *Jl = MODE == 0 ? nbl.j + (size_t)g * nbl.cap : nullptr;  // neighbor ids from the list
+    const int e0 = MODE == 0 ? 0 : env.env_start[g];
     for (int q = lane; q < n; q += WAVE) {
       double gx = S.fs[3 * q], gy = S.fs[3 * q + 1], gz = S.fs[3 * q + 2];
       own[0] += gx;
       own[1] += gy;
       own[2] += gz;
       if (MODE == 0) {
-        int j = S.j[q];
+        int j = Jl[q];
         unsafeAtomicAdd(&o.F[3 * (size_t)j], -gx);
         unsafeAtomicAdd(&o.F[3 * (size_t)j + 1], -gy);
         unsafeAtomicAdd(&o.F[3 * (size_t)j + 2], -gz);
       } else {
-        size_t e = (size_t)S.j[q];
+        size_t e = (size_t)(e0 + q);
         env.dF_j[3 * e] = -gx;
         env.dF_j[3 * e + 1] = -gy;
         env.dF_j[3 * e + 2] = -gz;
