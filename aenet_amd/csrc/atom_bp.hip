@@ -141,13 +141,18 @@ __device__ __forceinline__ void radial_fn(const BPFunc &f, float r, float fc, fl
 struct Trip {
   float uj[3], uk[3], d[3];  // d = R_k - R_j
   float rj, rk, rjk, cosv;
+  float ij, ik;  // 1 / rj, 1 / rk
 };
 
 __device__ __forceinline__ Trip make_trip(const float *R, const float *r, int ja, int kb) {
   Trip T;
   T.rj = r[ja];
   T.rk = r[kb];
-  const float ij = 1.0f / T.rj, ik = 1.0f / T.rk;
+  // hardware reciprocal and square root (1 ulp; FP32 geometry feeding FP32
+  // function values)
+  const float ij = __builtin_amdgcn_rcpf(T.rj), ik = __builtin_amdgcn_rcpf(T.rk);
+  T.ij = ij;
+  T.ik = ik;
   float dot = 0.0f, d2 = 0.0f;
 #pragma unroll
   for (int x = 0; x < 3; ++x) {
@@ -159,7 +164,7 @@ __device__ __forceinline__ Trip make_trip(const float *R, const float *r, int ja
   }
   dot = T.uj[0] * T.uk[0] + T.uj[1] * T.uk[1] + T.uj[2] * T.uk[2];
   T.cosv = fminf(1.0f, fmaxf(-1.0f, dot));
-  T.rjk = sqrtf(d2);
+  T.rjk = __builtin_amdgcn_sqrtf(d2);
   return T;
 }
 
@@ -361,8 +366,8 @@ struct AngForceOp {
       s_eta += wE * (-2.0f * eta);
     }
     if (c.rc > 0.0f) flush();
-    const float inv_rjk = T.rjk > 1.0e-8f ? 1.0f / T.rjk : 1.0f;
-    const float cj = dcos / T.rj, ck = dcos / T.rk;
+    const float inv_rjk = T.rjk > 1.0e-8f ? __builtin_amdgcn_rcpf(T.rjk) : 1.0f;
+    const float cj = dcos * T.ij, ck = dcos * T.ik;
 #pragma unroll
     for (int x = 0; x < 3; ++x) {
       const float e = dRjk * T.d[x] * inv_rjk;

@@ -45,9 +45,10 @@ __device__ __forceinline__ void cutoff(float r, float Rc, float inv_Rc, float &f
 // triangular index p -> (a, b), 0 <= a < b < n, row-major
 __device__ __forceinline__ void tri_decode(int p, int n, int &a, int &b) {
   float nn = (float)(2 * n - 1);
-  int r = (int)((nn - sqrtf(nn * nn - 8.0f * (float)p)) * 0.5f);
+  // hardware square root (1 ulp): the estimate stays within one row of the
+  // exact one (n <= 512), and the two steps below make the decode exact
+  int r = (int)((nn - __builtin_amdgcn_sqrtf(nn * nn - 8.0f * (float)p)) * 0.5f);
   r = min(max(r, 0), n - 2);
-  // the FP32 estimate is within one row of the exact one (n <= 512)
   if (r > 0 && r * n - r * (r + 1) / 2 > p) --r;
   if (r < n - 2 && (r + 1) * n - (r + 1) * (r + 2) / 2 <= p) ++r;
   a = r;
