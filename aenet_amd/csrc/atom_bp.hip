@@ -179,6 +179,7 @@ __device__ __forceinline__ void for_triplets(const BPWave &S, int b2, int m2, in
                                              float rc2, Op &op) {
   const int lane = lane_id();
   const int ncand = same ? m2 * (m2 - 1) / 2 : m2 * m3;
+  const float inv_m2 = m2 > 0 ? 1.0f / (float)m2 : 0.0f;
   const float inv_m3 = m3 > 0 ? 1.0f / (float)m3 : 0.0f;
   int qn = 0;
   for (int base = 0; base < ncand; base += WAVE) {
@@ -186,17 +187,30 @@ __device__ __forceinline__ void for_triplets(const BPWave &S, int b2, int m2, in
     bool ok = false;
     int packed = 0;
     if (p < ncand) {
+      // Diagonal orders: a run of consecutive candidates has distinct j and
+      // distinct k, so the force pass's per-neighbor atomics seldom meet at
+      // one address within a wavefront.
       int a, c;
       if (same) {
-        tri_decode(p, m2, a, c);
+        tri_decode_diag(p, m2, a, c);
       } else {
-        a = (int)(((float)p + 0.5f) * inv_m3);
-        c = p - a * m3;
+        // p = d m2 + a, c = (a + d) mod m3: for each a, d over [0, m3)
+        // takes c over [0, m3) once
+        int d = (int)(((float)p + 0.5f) * inv_m2);
+        a = p - d * m2;
+        if (a < 0) {
+          --d;
+          a += m2;
+        } else if (a >= m2) {
+          ++d;
+          a -= m2;
+        }
+        const int sum = a + d;
+        int q = (int)(((float)sum + 0.5f) * inv_m3);
+        c = sum - q * m3;
         if (c < 0) {
-          --a;
           c += m3;
         } else if (c >= m3) {
-          ++a;
           c -= m3;
         }
       }
