@@ -55,6 +55,7 @@ struct BPDev {
   int ang_p0[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];
   float ang_rcmax[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];  // largest cutoff of the block
   int ang_g5[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];       // block holds G5 (no r_jk cutoff)
+  int ang_one_seg[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];  // all functions: one cutoff, one kind
 };
 
 // one radial (G1/G2/G3) or angular (G4/G5) function record

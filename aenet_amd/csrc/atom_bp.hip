@@ -331,6 +331,7 @@ struct AngForceOp {
   bp_funcs fn;
   const float *coef;  // dE/dG of the block's functions
   int nf;
+  bool one_seg;       // every function of the block has one cutoff and one kind
   __device__ __forceinline__ void operator()(int ja, int kb, bool valid) {
     if (!valid) return;
     const Trip T = make_trip(R, r, ja, kb);
@@ -352,14 +353,7 @@ struct AngForceOp {
       if (seg_g4) dRjk += s_E * (c.cj * c.ck * c.dcjk) + s_eta * T.rjk * C;
       s_cos = s_E = s_eta = 0.0f;
     };
-    for (int k = 0; k < nf; ++k) {
-      const BPFunc f = load_fn(fn, k);
-      const bool g4 = f.kind == 4;
-      if (f.Rc != c.rc || g4 != seg_g4) {
-        if (c.rc > 0.0f) flush();
-        if (f.Rc != c.rc) trip_cut(T, f.Rc, c, tab, ja, kb);
-        seg_g4 = g4;
-      }
+    auto term = [&](const BPFunc &f, int k, bool g4) {
       const float lam = f.p1, zeta = f.p2, eta = f.p3;
       float arg, L;
       if (lam == 1.0f) {
@@ -378,6 +372,24 @@ struct AngForceOp {
       s_cos += w * (0.5f * zeta * lam) * P;
       s_E += wE;
       s_eta += wE * (-2.0f * eta);
+    };
+    if (one_seg) {
+      // one segment: cutoffs once, no per-function segment checks
+      const BPFunc f0 = load_fn(fn, 0);
+      seg_g4 = f0.kind == 4;
+      if (nf > 0) trip_cut(T, f0.Rc, c, tab, ja, kb);
+      for (int k = 0; k < nf; ++k) term(load_fn(fn, k), k, seg_g4);
+    } else {
+      for (int k = 0; k < nf; ++k) {
+        const BPFunc f = load_fn(fn, k);
+        const bool g4 = f.kind == 4;
+        if (f.Rc != c.rc || g4 != seg_g4) {
+          if (c.rc > 0.0f) flush();
+          if (f.Rc != c.rc) trip_cut(T, f.Rc, c, tab, ja, kb);
+          seg_g4 = g4;
+        }
+        term(f, k, g4);
+      }
     }
     if (c.rc > 0.0f) flush();
     const float inv_rjk = T.rjk > 1.0e-8f ? __builtin_amdgcn_rcpf(T.rjk) : 1.0f;
@@ -645,6 +657,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
         op.fn = bpf + bpt->ang_p0[key3];
         op.coef = coef + off;
         op.nf = nf;
+        op.one_seg = bpt->ang_one_seg[key3] != 0;
         for_triplets(S, b2, m2, b3, m3, t2 == t3, bpt->ang_g5[key3] != 0, rc * rc, op);
       }
     }

@@ -335,15 +335,19 @@ int PotSet::finalize(std::string &err) {
           bp.ang_n[key3] = (int)bpf.size() - bp.ang_p0[key3];
           bp.ang_rcmax[key3] = 0.0f;
           bp.ang_g5[key3] = 0;
+          bp.ang_one_seg[key3] = 1;
           for (int k = bp.ang_p0[key3]; k < (int)bpf.size(); ++k) {
             bp.ang_rcmax[key3] = std::max(bp.ang_rcmax[key3], bpf[k].Rc);
             if (bpf[k].kind == 5) bp.ang_g5[key3] = 1;
+            const BPFunc &f0 = bpf[bp.ang_p0[key3]];
+            if (bpf[k].Rc != f0.Rc || bpf[k].kind != f0.kind) bp.ang_one_seg[key3] = 0;
           }
           bp.ang_off[key3b] = bp.ang_off[key3];
           bp.ang_p0[key3b] = bp.ang_p0[key3];
           bp.ang_n[key3b] = bp.ang_n[key3];
           bp.ang_rcmax[key3b] = bp.ang_rcmax[key3];
           bp.ang_g5[key3b] = bp.ang_g5[key3];
+          bp.ang_one_seg[key3b] = bp.ang_one_seg[key3];
           off += bp.ang_n[key3];
         }
       }
