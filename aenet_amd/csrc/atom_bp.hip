@@ -378,6 +378,7 @@ struct AngForceOp {
       const BPFunc f0 = load_fn(fn, 0);
       seg_g4 = f0.kind == 4;
       if (nf > 0) trip_cut(T, f0.Rc, c, tab, ja, kb);
+#pragma unroll 4
       for (int k = 0; k < nf; ++k) term(load_fn(fn, k), k, seg_g4);
     } else {
       for (int k = 0; k < nf; ++k) {
