@@ -1,7 +1,7 @@
 /*
  * aenet_oracle.c -- FP64 CPU restatement of aenet's energy/force path.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path in aenet_amd/.
+ * TEST INFRASTRUCTURE ONLY: the tests hold the HIP path in aenet_amd/ to it.
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it; the product never links it.  Compiled with -ffp-contract=off so
  * that the FP64 neighbor arithmetic is evaluated term by term, in the order

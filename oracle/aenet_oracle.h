@@ -1,9 +1,9 @@
 /*
  * aenet_oracle.h -- FP64 CPU restatement of aenet's energy/force path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
- * implementation in aenet_amd/.  Only tests/, __graft_entry__.smoke() and
- * the cpu_baseline leg of bench.py may load it.  The product never links it.
+ * TEST INFRASTRUCTURE ONLY.  The tests hold the HIP implementation in
+ * aenet_amd/ to this library.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it.
  *
  * Every function follows the reference Fortran sources (ænet 2.0.4); the
  * file:line of the routine it restates is given next to each declaration.

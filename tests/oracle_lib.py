@@ -1,4 +1,5 @@
-"""ctypes access to the checkers (test infrastructure only).
+"""ctypes access to the reference implementations the tests compare against
+(test infrastructure only).
 
 * ``Oracle``  -- the FP64 C restatement in oracle/aenet_oracle.c
   (oracle/_build/liboracle.so).
@@ -6,7 +7,7 @@
   sources by ``make -C oracle ref`` (oracle/_ref/libaenet_ref.so), with
   oracle/refdrv.f90 driving it the way predict.x does.
 
-Both are checkers; the product (aenet_amd) never imports this module.
+Both are test references; the product (aenet_amd) never imports this module.
 """
 import ctypes as C
 import os
