@@ -19,6 +19,7 @@ AENET_ERR_TYPE = 4
 AENET_ERR_CAPACITY = 5
 AENET_ERR_DEVICE = 6
 AENET_ERR_ARG = 7
+AENET_ERR_COORD = 8
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -51,6 +52,8 @@ SIGNATURES = {
     "aenet_amd_batch_create": (None, [C.c_int, _ip, _dp, _ip, _dp, C.POINTER(_vp), _pint]),
     "aenet_amd_batch_destroy": (None, [_vp]),
     "aenet_amd_batch_natoms": (C.c_int, [_vp]),
+    "aenet_amd_batch_replan": (None, [_vp, _dp, _pint]),
+    "aenet_amd_batch_has_isolated": (C.c_int, [_vp]),
     "aenet_amd_batch_eval": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
     "aenet_amd_batch_check": (None, [_vp, _vp, _pint]),
     "aenet_amd_batch_set_timing": (None, [_vp, C.c_int]),

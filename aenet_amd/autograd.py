@@ -14,17 +14,31 @@ import torch
 
 
 class _PotentialEnergy(torch.autograd.Function):
+    """The weights an evaluation used are kept with it: backward
+    differentiates at those weights, whatever the model holds by then, and
+    neither pass leaves the model's weights changed."""
+
     @staticmethod
     def forward(ctx, coords, weights, batch, model):
-        if model is not None and weights is not None:
-            model.set_weights(weights)
+        W_used = None
+        if model is not None:
+            W_prev = model.weights()
+            W_used = W_prev if weights is None else weights.detach().to(torch.float32).contiguous().clone()
         x = coords.detach().to(torch.float64).contiguous().view(-1)
-        E_tot, _, E_at, _ = batch.eval(coords=x, forces=False, check=True, model=model)
+        if weights is not None and model is not None:
+            model.set_weights(W_used)
+        try:
+            E_tot, _, E_at, _ = batch.eval(coords=x, forces=False, check=True, model=model)
+            E_tot, E_at = E_tot.clone(), E_at.clone()
+        finally:
+            if weights is not None and model is not None:
+                model.set_weights(W_prev)
         ctx.batch = batch
         ctx.model = model
+        ctx.W_used = W_used
         ctx.has_w = weights is not None
         ctx.save_for_backward(x)
-        return E_tot.clone(), E_at.clone()
+        return E_tot, E_at
 
     @staticmethod
     def backward(ctx, g_tot, g_at):
@@ -39,15 +53,23 @@ class _PotentialEnergy(torch.autograd.Function):
         gW = None
         if ctx.has_w and ctx.needs_input_grad[1]:
             gW = torch.zeros(model.wsize, dtype=torch.float32, device=x.device)
-        gR = batch.vjp(g.contiguous(), grad_W=gW, coords=x, model=model)
+        W_now = None
+        if model is not None:
+            W_now = model.weights()
+            model.set_weights(ctx.W_used)
+        try:
+            gR = batch.vjp(g.contiguous(), grad_W=gW, coords=x, model=model)
+        finally:
+            if W_now is not None:
+                model.set_weights(W_now)
         return gR.view(-1, 3), gW, None, None
 
 
 def energies(batch, coords, weights=None, model=None):
     """(E_tot per structure, E_atoms) of the batch at coords ((natoms, 3)
     float64 device tensor), differentiable with respect to coords and, when
-    given, the model's weights (a float32 tensor of the model's weight
-    layout; it is loaded into the model before the evaluation)."""
+    given, the weights (a float32 tensor of the model's weight layout; the
+    model evaluates with it and gets its own weights back afterwards)."""
     return _PotentialEnergy.apply(coords, weights, batch, model if model is not None else batch.model)
 
 

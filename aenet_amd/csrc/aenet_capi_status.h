@@ -10,4 +10,5 @@
 #define AENET_STATUS_ERR_CAPACITY 5   // a neighbor/cell capacity was exceeded
 #define AENET_STATUS_ERR_DEVICE 6     // HIP runtime error or no usable GPU
 #define AENET_STATUS_ERR_ARG 7        // invalid argument
+#define AENET_STATUS_ERR_COORD 8      // non-finite coordinates, or an isolated structure left its bounding box
 #define AENET_AMD_ERR_ARG_VALUE AENET_STATUS_ERR_ARG

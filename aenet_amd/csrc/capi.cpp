@@ -629,13 +629,20 @@ void aenet_nbl_neighbors(int iatom, int *nnb, double nbcoo[], double nbdist[], i
     return;
   }
   int n = g.nbl_count[iatom - 1];
-  int m = std::min(n, *nnb);
+  int m = std::min(n, std::max(*nnb, 0));
   size_t base = (size_t)(iatom - 1) * g.nbl_cap;
   std::memcpy(nbcoo, &g.nbl_coo[3 * base], 3 * (size_t)m * sizeof(double));
   std::memcpy(nbdist, &g.nbl_dist[base], (size_t)m * sizeof(double));
   std::memcpy(nblist, &g.nbl_list[base], (size_t)m * sizeof(int));
   std::memcpy(nbtype, &g.nbl_type[base], (size_t)m * sizeof(int));
-  *nnb = n;
+  // the reference stops the program when the caller's arrays are too
+  // small (src/ext/lclist.f90:552-556, :653-660); here *nnb is the number
+  // of entries written and the error is recorded for aenet_amd_last_error
+  if (m < n)
+    set_err(AENET_STATUS_ERR_CAPACITY, "aenet_nbl_neighbors: atom " + std::to_string(iatom) + " has " +
+                                           std::to_string(n) + " neighbors, room for " + std::to_string(*nnb) +
+                                           "; the list is truncated");
+  *nnb = m;
 }
 
 // ---------------------------------------------------------------------
@@ -722,13 +729,33 @@ void aenet_amd_batch_create(int nstruct, const int natoms[], const double latt[]
 
 void aenet_amd_batch_destroy(aenet_amd_batch *batch) { delete batch; }
 
+void aenet_amd_batch_replan(aenet_amd_batch *batch, const double coords_host[], int *stat) {
+  if (!batch || !coords_host) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_amd_batch_replan: no batch or no coordinates");
+    return;
+  }
+  std::string err;
+  int st = batch->batch.replan(coords_host, err);
+  *stat = st ? set_err(st, err) : AENET_STATUS_OK;
+}
+
+int aenet_amd_batch_has_isolated(const aenet_amd_batch *batch) {
+  return batch && batch->batch.has_isolated() ? 1 : 0;
+}
+
 int aenet_amd_batch_natoms(const aenet_amd_batch *batch) { return batch ? batch->batch.natoms() : 0; }
 
 void aenet_amd_batch_eval(aenet_amd_batch *batch, const double *d_coords, const int *d_types, double *d_E_atoms,
                           double *d_E_struct, double *d_F, void *stream, int *stat) {
+  // the lock keeps aenet_final from releasing the potentials during the launch
+  std::lock_guard<std::mutex> lk(g_mu);
   Global &g = G();
   if (!batch || !g.dev) {
     *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or potentials not loaded");
+    return;
+  }
+  if (!batch->batch.fits(*g.dev)) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "potential cutoff exceeds the cutoff the batch was created for");
     return;
   }
   hipStream_t s = (hipStream_t)stream;  // NULL: the default stream

@@ -97,8 +97,43 @@ int Batch::create(const PotSet &ps, int nstruct, const int *natoms, const double
   return allocate(err);
 }
 
+bool Batch::has_isolated() const {
+  if (lattice_coords_) return false;
+  for (const StructDev &d : plan_.st)
+    if (!d.pbc) return true;
+  return false;
+}
+
+int Batch::replan(const double *coords, std::string &err) {
+  if (!has_isolated()) return AENET_STATUS_OK;
+  const int ns = (int)plan_.st.size();
+  std::vector<int> natoms(ns), pbc(ns);
+  std::vector<double> latt(9 * (size_t)ns);
+  for (int s = 0; s < ns; ++s) {
+    natoms[s] = plan_.st[s].natoms;
+    pbc[s] = plan_.st[s].pbc;
+    std::memcpy(&latt[9 * (size_t)s], plan_.st[s].L, 9 * sizeof(double));
+  }
+  for (size_t i = 0; i < 3 * (size_t)plan_.natoms; ++i)
+    if (!std::isfinite(coords[i])) {
+      err = "non-finite coordinates";
+      return AENET_STATUS_ERR_COORD;
+    }
+  PlanHost np;
+  int st = build_plan(rc_max_, ns, natoms.data(), latt.data(), pbc.data(), coords, np, err);
+  if (st) return st;
+  const int cap = nb_cap_;
+  plan_ = std::move(np);
+  nb_cap_ = 0;
+  st = allocate(err);
+  if (st) return st;
+  // keep a list capacity that an earlier evaluation needed
+  return cap > nb_cap_ ? set_nb_cap(cap) : AENET_STATUS_OK;
+}
+
 int Batch::create_lattice_coords(const PotSet &ps, int natoms, const double *latt, int pbc, std::string &err) {
   rc_max_ = ps.Rc_max;
+  lattice_coords_ = true;
   int st = build_plan_lattice(ps.Rc_max, natoms, latt, pbc, plan_, err);
   if (st) return st;
   return allocate(err);
@@ -503,7 +538,8 @@ int Batch::check_flags(hipStream_t stream, std::string &err) {
     err = "kernel reported:" + flag_text(f);
     int zero = 0;
     (void)hipMemcpy(flags_.p, &zero, sizeof(int), hipMemcpyHostToDevice);
-    return (f & AENET_AMD_FLAG_BAD_TYPE) ? AENET_STATUS_ERR_TYPE : AENET_STATUS_ERR_CAPACITY;
+    if (f & AENET_AMD_FLAG_BAD_TYPE) return AENET_STATUS_ERR_TYPE;
+    return (f & ~AENET_AMD_FLAG_BAD_COORD) ? AENET_STATUS_ERR_CAPACITY : AENET_STATUS_ERR_COORD;
   }
   return AENET_STATUS_OK;
 }

@@ -68,6 +68,11 @@ class Batch {
              const double *coords, std::string &err);
   // one structure given in lattice coordinates of latt (aenet_nbl_init)
   int create_lattice_coords(const PotSet &ps, int natoms, const double *latt, int pbc, std::string &err);
+  // new bounding boxes of the isolated structures from coordinates (host,
+  // Cartesian), as geo_get_bounds takes them; periodic structures keep
+  // their plans.  The measured launch shapes are kept.
+  int replan(const double *coords, std::string &err);
+  bool has_isolated() const;
   const double *wrapped_coords() const { return s_.p; }
   // all pointers are device pointers; E_struct receives 2 per structure
   // (total, cohesive); F may be null when forces are not wanted
@@ -125,6 +130,7 @@ class Batch {
   int allocate(std::string &err);
   PlanHost plan_;
   double rc_max_ = 0.0;
+  bool lattice_coords_ = false;
   std::vector<double> occ_;  // occupied volume fraction of each structure (density estimate)
   DBuf<StructDev> st_;
   DBuf<int> atom_struct_, cell_struct_, cell_ijk_, Tvec_, Cvec_, self_t_;

@@ -69,18 +69,30 @@ class DeviceBatch:
 
     def eval(self, coords=None, types=None, forces=True, check=True, model=None):
         """Launch one evaluation on the current torch stream; returns
-        (E_tot, E_coh, E_atoms, forces) device tensors."""
+        (E_tot, E_coh, E_atoms, forces) device tensors.  With check, an
+        isolated structure whose atoms left the bounding box of its plan
+        gets a new plan from the current coordinates and the evaluation is
+        repeated (without check, keep isolated structures inside their
+        boxes, e.g. under graph replay)."""
         coords = self.coords if coords is None else coords
         types = self.types if types is None else types
         self._check_inputs(coords, types)
-        stat = C.c_int(0)
-        _native.lib().aenet_amd_batch_eval_model(self._h, self._model(model), _ptr(coords), _ptr(types),
-                                                 _ptr(self.E_atoms), _ptr(self.E_struct),
-                                                 _ptr(self.forces) if forces else None, self._stream(),
-                                                 C.byref(stat))
-        check_status(stat.value, "aenet_amd_batch_eval")
-        if check:
-            self.check()
+        for attempt in range(2):
+            stat = C.c_int(0)
+            _native.lib().aenet_amd_batch_eval_model(self._h, self._model(model), _ptr(coords), _ptr(types),
+                                                     _ptr(self.E_atoms), _ptr(self.E_struct),
+                                                     _ptr(self.forces) if forces else None, self._stream(),
+                                                     C.byref(stat))
+            check_status(stat.value, "aenet_amd_batch_eval")
+            if not check:
+                break
+            stat = C.c_int(0)
+            _native.lib().aenet_amd_batch_check(self._h, self._stream(), C.byref(stat))
+            if stat.value == _native.AENET_ERR_COORD and attempt == 0 and self.has_isolated():
+                self.replan(coords)
+                continue
+            check_status(stat.value, "aenet_amd_batch_check")
+            break
         return (self.E_struct[:self.nstruct], self.E_struct[self.nstruct:], self.E_atoms,
                 self.forces if forces else None)
 
@@ -99,7 +111,8 @@ class DeviceBatch:
 
     def vjp(self, g, grad_W=None, coords=None, types=None, model=None):
         """sum_i g_i dE_i/dR (returned, (natoms, 3) float64) and, when grad_W
-        is given, grad_W += sum_i g_i dE_i/dW."""
+        is given, grad_W += sum_i g_i dE_i/dW.  The coordinates must lie in
+        the isolated structures' boxes (as after a checked eval of them)."""
         coords = self.coords if coords is None else coords
         types = self.types if types is None else types
         self._check_inputs(coords, types)
@@ -110,6 +123,19 @@ class DeviceBatch:
                                           _ptr(grad_W), self._stream(), C.byref(stat))
         check_status(stat.value, "aenet_amd_batch_vjp")
         return gR
+
+    def has_isolated(self):
+        return bool(_native.lib().aenet_amd_batch_has_isolated(self._h))
+
+    def replan(self, coords=None):
+        """New bounding boxes of the isolated structures from coordinates
+        (default: the batch's own tensor); periodic structures keep their
+        plans (aenet_amd_batch_replan)."""
+        coords = self.coords if coords is None else coords
+        x = np.ascontiguousarray(coords.detach().to("cpu", torch.float64).numpy().reshape(-1))
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_replan(self._h, x, C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_replan")
 
     def check(self):
         stat = C.c_int(0)

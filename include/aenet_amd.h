@@ -6,7 +6,7 @@
  * list, descriptors, network and forces for whole structures -- into one
  * device pass.  They use the potentials loaded with aenet_init /
  * aenet_load_potential (include/aenet.h).  Status codes are the AENET_*
- * values; codes 5-7 are this library's own (see AENET_AMD_ERR_*).
+ * values; codes 5-8 are this library's own (see AENET_AMD_ERR_*).
  *
  * Conventions: lattices are 9 doubles per structure, vector k at
  * [3k, 3k+1, 3k+2] (the column-major latticeVec(3,3) of the reference, or
@@ -25,6 +25,8 @@ extern "C" {
 #define AENET_AMD_ERR_CAPACITY 5
 #define AENET_AMD_ERR_DEVICE 6
 #define AENET_AMD_ERR_ARG 7
+#define AENET_AMD_ERR_COORD 8 /* non-finite coordinates, or an isolated
+                                 structure left its bounding box */
 
 /* Energies and forces of nstruct structures given in host memory.
  * E_tot/E_coh: per structure (may be NULL); E_atoms: per atom (may be
@@ -47,6 +49,18 @@ void aenet_amd_batch_create(int nstruct, const int natoms[], const double latt[]
                             aenet_amd_batch **batch, int *stat);
 void aenet_amd_batch_destroy(aenet_amd_batch *batch);
 int aenet_amd_batch_natoms(const aenet_amd_batch *batch);
+
+/* Isolated structures (pbc = 0) are binned in the bounding box of the
+ * coordinates given at creation (geo_get_bounds).  An evaluation whose
+ * device coordinates leave that box raises AENET_AMD_ERR_COORD at the next
+ * aenet_amd_batch_check; until then (unchecked calls, graph replays) atoms
+ * outside the box are binned into its edge cells and neighbors beyond
+ * them can be missed.  aenet_amd_batch_replan makes new bounding boxes
+ * from host coordinates (periodic structures are unchanged); the
+ * evaluation is then repeated.  has_isolated: 1 if the batch has any
+ * isolated structure. */
+void aenet_amd_batch_replan(aenet_amd_batch *batch, const double coords_host[], int *stat);
+int aenet_amd_batch_has_isolated(const aenet_amd_batch *batch);
 
 /* d_* are device pointers; stream is the hipStream_t every entry point
  * with device pointers launches on (NULL: the default stream, as in HIP),

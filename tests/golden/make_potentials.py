@@ -1,6 +1,7 @@
 """Write the synthetic ænet potentials the workloads use (tests/golden/potentials/).
 
 Run from the repository root:  python tests/golden/make_potentials.py
+(``... make_potentials.py mixed`` writes only the mixed Behler-Parrinello set)
 
 Weights are drawn N(0, 1/fan_in) with fixed seeds (fan_in = number of nodes
 of the previous layer); normalization statistics (sfval_avg = mean,
@@ -117,8 +118,60 @@ def bp_functions(types):
     return fns
 
 
+def mixed_functions(types):
+    """Behler-Parrinello functions of every kind (G1-G5) with mixed cutoffs:
+    G2 with Rs = 0.5 at 4.5 and 6.0 A, G3 at 5.5 A, and G4/G5 of one type
+    pair at 4.5, 5.0 and 6.0 A."""
+    fns = []
+    for t2 in types:
+        fns.append({"G": 1, "type2": t2, "Rc": 5.0})
+        for eta, rc in ((0.01, 6.0), (0.1, 6.0), (0.05, 4.5)):
+            fns.append({"G": 2, "type2": t2, "Rc": rc, "Rs": 0.5, "eta": eta})
+        fns.append({"G": 3, "type2": t2, "Rc": 5.5, "kappa": 1.3})
+    for a in range(len(types)):
+        for b in range(a, len(types)):
+            for lam, zeta, eta, rc in ((1.0, 1.0, 0.003, 6.0), (-1.0, 2.0, 0.01, 6.0),
+                                       (1.0, 4.0, 0.02, 4.5), (-1.0, 1.0, 0.005, 4.5)):
+                fns.append({"G": 4, "type2": types[a], "type3": types[b], "Rc": rc,
+                            "lambda": lam, "zeta": zeta, "eta": eta})
+            for lam, zeta, eta, rc in ((1.0, 2.0, 0.01, 5.0), (-1.0, 1.0, 0.002, 6.0)):
+                fns.append({"G": 5, "type2": types[a], "type3": types[b], "Rc": rc,
+                            "lambda": lam, "zeta": zeta, "eta": eta})
+    return fns
+
+
+def mixed_structures():
+    """The two structures the mixed set is checked on: a 108-atom binary
+    bulk cell and a 40-atom isolated cluster."""
+    bulk = synth.binary_fcc(3.7, (3, 3, 3), 0.1, 41, 42)
+    bulk.label = "bulk108"
+    _, big = synth.fcc(3.6, (6, 6, 6))
+    big = big - big.mean(axis=0)
+    pos = big[np.argsort(np.sum(big * big, axis=1), kind="stable")[:40]]
+    rng = np.random.Generator(np.random.PCG64(43))
+    pos = pos + rng.normal(0.0, 0.1, size=pos.shape)
+    cluster = synth.Structure(np.eye(3), pos, rng.integers(1, 3, len(pos)).astype(np.int32), False,
+                              "cluster40")
+    return {"bulk108": bulk, "cluster40": cluster}
+
+
+def mixed_set():
+    types = ["A", "B"]
+    fns = mixed_functions(types)
+    pots = {}
+    for k, t in enumerate(types):
+        st = nnfile.behler_setup(t, types, fns)
+        pots[t] = make_pot(t, st, [st["nsf"], 12, 1], 60 + k, types, [-1000.0, -1500.0])
+    calibrate(pots, types, list(mixed_structures().values()))
+    return write_set("mixed", types, pots)
+
+
 def main():
     os.makedirs(OUT, exist_ok=True)
+    if sys.argv[1:] == ["mixed"]:
+        mixed_set()
+        print("wrote the mixed set to", OUT)
+        return
     # fcc108: single species, Chebyshev radial 16 / 6.0 A, angular 4 / 4.0 A, 22-20-20-1
     c, p = synth.fcc(3.61, (3, 3, 3))
     cal = [synth.Structure(c, synth.displaced(p, 0.1, 3), np.ones(len(p), np.int32))]
@@ -145,6 +198,8 @@ def main():
         cal.append(synth.Structure(c, synth.displaced(p, 0.1, seed + 10), rng.integers(1, 4, len(p))))
     calibrate(pots, types, cal)
     write_set("bp3", types, pots)
+    # Behler-Parrinello with every function kind and mixed cutoffs, 2 species
+    mixed_set()
     print("wrote potentials to", OUT)
 
 

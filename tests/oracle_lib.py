@@ -220,6 +220,8 @@ class Reference:
                                            C.c_double, C.POINTER(C.c_int)]
             lib.aenet_sfb_final.argtypes = [C.POINTER(C.c_int)]
             lib.aenet_sfb_eval.argtypes = [C.c_int, _dp, C.c_int, _ip, _dp, C.c_int, _dp, C.POINTER(C.c_int)]
+            lib.aenet_sfb_nvalues.restype = C.c_int
+            lib.aenet_sfb_reconstruct_radial.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp, C.POINTER(C.c_int)]
             lib.refdrv_cart_to_frac.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp, _dp]
             lib.refdrv_neighbors.argtypes = [_dp, C.c_int, C.c_int, _ip, _dp, C.c_int, _ip, _dp, _dp, _ip,
                                              _ip, _ip]

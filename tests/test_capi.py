@@ -77,6 +77,13 @@ def test_error_paths_follow_reference():
         L.aenet_load_potential(1, TI_O["Ti"].encode(), C.byref(st))
         assert st.value == 0
         assert not L.aenet_all_loaded()
+        # free-atom energy of a loaded species (src/aenet.f90:337-346): the
+        # E_atom of the training set record (src/potential.f90:150-151)
+        from aenet_amd import nnfile
+        p = nnfile.read_nn(TI_O["Ti"])
+        e_ti = p["ts_E_atom"][p["ts_types"].index("Ti")]
+        assert L.aenet_free_atom_energy(1) == e_ti
+        assert np.isnan(L.aenet_free_atom_energy(2))      # not loaded yet
         L.aenet_atomic_energy(np.zeros(3), 1, 0, np.zeros(3), np.zeros(1, np.int32), E, C.byref(st))
         assert st.value == _native.AENET_ERR_INIT
         # type conversion by name (geo_type_conv, src/geometry.f90:440-461)

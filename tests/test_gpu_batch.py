@@ -175,3 +175,42 @@ def test_autotune_keeps_results(device):
             assert (F1 - F0).abs().max().item() < 1e-10
         finally:
             ann.release()
+
+
+def test_isolated_structure_leaving_its_box_is_replanned(device):
+    """An isolated cluster in a device batch whose atom moves outward, past
+    the bounding box the batch was planned with: the checked evaluation
+    makes a new plan from the current coordinates and repeats, and the
+    energies and forces equal the oracle's for the moved cluster (which
+    takes the bounding box of the moved coordinates, geo_get_bounds)."""
+    import torch
+    from aenet_amd import synth
+    from aenet_amd.core import ANNPotentials
+    from aenet_amd.device import DeviceBatch
+    from oracle_lib import Oracle
+    files = pot_files("bp3", NAMES)
+    ann = ANNPotentials(files)
+    try:
+        cl = [s for s in synth.batch_uneven(40) if not s.pbc][0]
+        bulk = [s for s in synth.batch_uneven(40) if s.pbc][0]
+        db = DeviceBatch([bulk, cl], device=device)
+        db.eval()
+        moved = cl.positions.copy()
+        far = int(np.argmax(np.sum((moved - moved.mean(axis=0)) ** 2, axis=1)))
+        d = moved[far] - moved.mean(axis=0)
+        moved[far] += 1.5 * d / np.linalg.norm(d)      # 1.5 A outward: outside the old box
+        x = db.coords.clone().view(-1, 3)
+        n0 = len(bulk.types)
+        x[n0:] = torch.as_tensor(moved, device=device)
+        print("case cluster", len(cl.types), "atoms, atom", far, "moved out", flush=True)
+        E_tot, _, E_at, F = db.eval(coords=x.view(-1).contiguous())
+        o = Oracle(NAMES, files)
+        ref = o.structure(cl.cell, False, cl.types, moved)
+        dE = np.abs(E_at[n0:].cpu().numpy() - ref["energies"]).max()
+        dF = np.abs(F[n0:].cpu().numpy() - ref["forces"]).max()
+        print("moved cluster: max |dE| %.3e eV/atom  max |dF| %.3e eV/A" % (dE, dF))
+        assert dE < 1e-5 and dF < 1e-4
+        ref_b = o.structure(bulk.cell, True, bulk.types, bulk.positions)
+        assert np.abs(E_at[:n0].cpu().numpy() - ref_b["energies"]).max() < 1e-5
+    finally:
+        ann.release()

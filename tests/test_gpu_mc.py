@@ -114,6 +114,41 @@ def test_fit_gradient_and_training(device):
     assert e1 ** 2 + f1 ** 2 < e0 ** 2 + f0 ** 2
 
 
+def test_fit_forward_matches_reference(device):
+    """The fit kernel's own forward pass (the student's atomic energies and
+    forces inside its loss) against the reference build's values for the
+    student potential (tests/golden/ref_07_student_student.npz, recorded
+    from aenet_atomic_energy_and_forces, src/aenet.f90:410-486).  With the
+    reference values as targets and zero loss weights, the kernel's loss
+    sums are sum_i (E_i - E_ref,i)^2 and sum |F - F_ref|^2; bounding each
+    sum by the square of the per-atom bar bounds every atom's error by the
+    bar.  The gradient stays zero."""
+    import os
+    import torch
+    from aenet_amd.model import Model
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "ref_07_student_student.npz"), allow_pickle=False)
+    assert [str(t) for t in g["types"]] == AB and bool(g["pbc"])
+    n = len(g["atom_types"])
+    # the fit kernel uses minimum images: the cell must be wider than 2 Rc
+    assert np.min(np.diag(g["cell"])) > 2 * 5.0
+    student = Model(pot_files("student", AB), device=device)
+    coords = torch.as_tensor(g["positions"].reshape(-1), dtype=torch.float64, device=device)
+    types = torch.as_tensor(g["atom_types"], dtype=torch.int32, device=device)
+    E_ref = torch.as_tensor(g["energies"], dtype=torch.float64, device=device)
+    F_ref = torch.as_tensor(g["forces"].reshape(-1), dtype=torch.float64, device=device)
+    sel = torch.zeros(1, dtype=torch.int32, device=device)
+    grad = torch.zeros(student.wsize, dtype=torch.float32, device=device)
+    loss = torch.zeros(2, dtype=torch.float64, device=device)
+    student.fit_cells(g["cell"], n, coords, types, sel, E_ref, F_ref, 0.0, 0.0, grad, loss)
+    L = loss.cpu().numpy()
+    print("fit forward vs reference: sqrt(sum dE^2) %.3e eV, sqrt(sum |dF|^2) %.3e eV/A (over %d atoms)"
+          % (np.sqrt(L[0]), np.sqrt(L[1]), n))
+    assert np.abs(g["forces"]).max() > 0.1
+    assert np.sqrt(L[0]) < 1e-5   # implies every |dE_i| < 1e-5 eV
+    assert np.sqrt(L[1]) < 1e-4   # implies every force component within 1e-4 eV/A
+    assert not grad.any()
+
+
 def _rule_check(p, hit, label):
     """Bernoulli outcomes hit with probabilities p: total and per p-bin
     counts within 4 standard deviations."""

@@ -247,3 +247,103 @@ def test_bp_descriptors_match_oracle(case, device):
         worst = max(worst, float(err.max()))
     print("max relative |dG| %.3e" % worst)
     assert worst < 1e-5
+
+
+def test_alloy4000_full_size(device):
+    """The 4000-atom alloy (the headline workload) at full size against the
+    reference build (tests/golden/full_alloy4000.npz): the 5^3 cell grid,
+    the pruned stencil and the cell-major work list.  Every atom's energy
+    and force within the bars; neighbor lists of every 40th atom bit-exact."""
+    from aenet_amd import synth
+    from aenet_amd.core import ANNPotentials
+    from aenet_amd.device import DeviceBatch
+    g = _load(os.path.join(os.path.dirname(__file__), "golden", "full_alloy4000.npz"))
+    s = synth.c2_alloy4000()
+    assert np.array_equal(s.positions, g["positions"]) and np.array_equal(s.types, g["atom_types"])
+    ann = ANNPotentials(_pots(g))
+    try:
+        r = ann.structure_energy_and_forces(g["cell"], True, g["positions"], list(g["atom_types"]))
+        dE = np.abs(r["energies"] - g["energies"]).max()
+        dF = np.abs(r["forces"] - g["forces"]).max()
+        print("alloy4000: max |dE| %.3e eV/atom  max |dF| %.3e eV/A  dE_tot %.3e eV"
+              % (dE, dF, abs(r["E_tot"][0] - g["Etot"])), flush=True)
+        assert dE < E_TOL
+        assert dF < F_TOL
+        db = DeviceBatch([_S(g)], device=device)
+        nb = db.neighbors(g["nb_coo"].shape[1] + 8)
+        nnb = nb["nnb"].cpu().numpy()
+        assert np.array_equal(nnb, g["nnb"])
+        for row, i in enumerate(g["nb_atoms"]):
+            n = nnb[i]
+            assert np.array_equal(nb["coo"][i, :n].cpu().numpy(), g["nb_coo"][row, :n])
+            assert np.array_equal(nb["dist"][i, :n].cpu().numpy(), g["nb_dist"][row, :n])
+            assert np.array_equal(nb["list"][i, :n].cpu().numpy(), g["nb_list"][row, :n])
+            assert np.array_equal(nb["type"][i, :n].cpu().numpy(), g["nb_type"][row, :n])
+        del db
+    finally:
+        ann.release()
+
+
+def test_sfb_reconstruct_radial_matches_reference(device):
+    """aenet_sfb_eval and aenet_sfb_reconstruct_radial against the reference
+    build's values for the same environment (tests/golden/sfb_reference.npz,
+    src/aenet.f90:749-799 and src/ext/sfbasis.f90:581-637).  The reference
+    returns NaN at r = 0 (its weight 1/sqrt(r - r^2)); so does this one."""
+    import ctypes as C
+    from aenet_amd import _native
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "sfb_reference.npz"), allow_pickle=False)
+    L = _native.lib()
+    stat = C.c_int(0)
+    L.aenet_sfb_init(2, (C.c_char_p * 2)(b"A", b"B"), int(g["radial_order"]), int(g["angular_order"]),
+                     float(g["radial_Rc"]), float(g["angular_Rc"]), C.byref(stat))
+    assert stat.value == 0, _native.last_error()
+    try:
+        nv = L.aenet_sfb_nvalues()
+        vals = np.zeros(nv)
+        L.aenet_sfb_eval(1, np.zeros(3), len(g["types"]), np.ascontiguousarray(g["types"]),
+                         np.ascontiguousarray(g["coo"]).reshape(-1), nv, vals, C.byref(stat))
+        assert stat.value == 0, _native.last_error()
+        assert np.allclose(vals, g["values"], rtol=2e-6, atol=2e-5)
+        nx = int(g["nx"])
+        x = np.zeros(nx)
+        y = np.zeros(nx)
+        L.aenet_sfb_reconstruct_radial(nv, np.ascontiguousarray(g["values"]), nx, x, y, C.byref(stat))
+        assert stat.value == 0, _native.last_error()
+        assert np.array_equal(x, g["x"])
+        assert np.isnan(y[0]) and np.isnan(g["y"][0])
+        assert np.allclose(y[1:], g["y"][1:], rtol=1e-12, atol=1e-12)
+    finally:
+        L.aenet_sfb_final(C.byref(stat))
+
+
+def test_nbl_neighbors_small_buffer_is_refused(device):
+    """aenet_nbl_neighbors with room for fewer entries than the atom has:
+    the reference stops the program (src/ext/lclist.f90:552-556); here the
+    entries that fit are written, *nnb says how many, and the error is
+    recorded."""
+    import ctypes as C
+    from aenet_amd import _native
+    from aenet_amd.core import ANNPotentials
+    g = _load(FIXTURES[0])
+    ann = ANNPotentials(_pots(g))
+    try:
+        L = _native.lib()
+        n = len(g["atom_types"])
+        coords = np.ascontiguousarray(g["frac"]).reshape(-1).copy()
+        L.aenet_nbl_init(np.ascontiguousarray(g["latt"]).reshape(-1), n, np.ascontiguousarray(g["atom_types"]),
+                         coords, False, bool(g["pbc"]))
+        full = int(g["nnb"][0])
+        room = 10
+        assert full > room
+        nnb = C.c_int(room)
+        coo = np.zeros(3 * room)
+        dist = np.zeros(room)
+        lst = np.zeros(room, np.int32)
+        typ = np.zeros(room, np.int32)
+        L.aenet_nbl_neighbors(1, C.byref(nnb), coo, dist, lst, typ)
+        assert nnb.value == room
+        assert "truncated" in _native.last_error()
+        assert np.array_equal(dist, g["nb_dist"][0, :room])
+        L.aenet_nbl_final()
+    finally:
+        ann.release()

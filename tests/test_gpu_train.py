@@ -193,3 +193,33 @@ def test_autograd_forces_and_weights(device):
         print("dE/dW[%d] = %.6f  fd %.6f" % (k, gWn[k], fd))
         assert abs(fd - gWn[k]) <= 5e-3 * abs(gWn[k]) + 1e-3
     m.set_weights(W0)
+
+
+def test_autograd_keeps_the_forward_weights(device):
+    """Two forwards at different weights, then backward of the first:
+    gradients are those at the first forward's weights, and the model's
+    own weights are unchanged by either pass."""
+    import torch
+    from aenet_amd import autograd
+    m, db = _model_and_batch(device)
+    x = db.coords.view(-1, 3).clone()
+    W0 = m.weights()
+    W1 = (W0 * 1.05).clone().requires_grad_(True)
+    W2 = (W0 * 0.9).clone().requires_grad_(True)
+    xa = x.clone().requires_grad_(True)
+    Ea, _ = autograd.energies(db, xa, weights=W1, model=m)
+    Eb, _ = autograd.energies(db, x.clone().requires_grad_(True), weights=W2, model=m)
+    assert torch.equal(m.weights(), W0)
+    assert abs(Ea.item() - Eb.item()) > 1e-3
+    gxa, gWa = torch.autograd.grad(Ea.sum(), (xa, W1))
+    assert torch.equal(m.weights(), W0)
+    # reference: the same quantities from a single forward at W1
+    xr = x.clone().requires_grad_(True)
+    W1r = W1.detach().clone().requires_grad_(True)
+    Er, _ = autograd.energies(db, xr, weights=W1r, model=m)
+    gxr, gWr = torch.autograd.grad(Er.sum(), (xr, W1r))
+    print("two-forward backward vs single: |dgx| %.3e |dgW| %.3e"
+          % ((gxa - gxr).abs().max().item(), (gWa - gWr).abs().max().item()))
+    assert torch.equal(Ea, Er)
+    assert torch.allclose(gxa, gxr, rtol=0, atol=1e-9)
+    assert torch.allclose(gWa, gWr, rtol=1e-6, atol=1e-6)

@@ -18,6 +18,7 @@ import pytest
 from conftest import TI2O4_CELL, TI2O4_POS, TI2O4_TYPES, TI_O, pot_files
 from oracle_lib import Oracle, Reference
 from aenet_amd import synth
+from golden.make_potentials import mixed_structures
 
 needs_ref = pytest.mark.skipif(not Reference.available(),
                                reason="reference library not built (make -C oracle ref needs /root/reference)")
@@ -31,6 +32,9 @@ def _structures():
         out.append(("bp3", ["A", "B", "C"], b[i]))
     out.append(("teacher", ["A", "B"], synth.binary_fcc(3.61, (3, 3, 3), 0.1, 5, 6)))
     out.append(("student", ["A", "B"], synth.binary_fcc(3.61, (3, 3, 3), 0.1, 5, 6)))
+    # Behler-Parrinello G1-G5, G2 with Rs = 0.5, radii 4.5 / 5.0 / 5.5 / 6.0 A
+    for s in mixed_structures().values():
+        out.append(("mixed", ["A", "B"], s))
     return out
 
 
