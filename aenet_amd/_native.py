@@ -49,6 +49,8 @@ SIGNATURES = {
     "aenet_nbl_neighbors": (None, [C.c_int, _pint, _dp, _dp, _ip, _ip]),
     "aenet_structures_energy_and_forces": (None, [C.c_int, _ip, _dp, _ip, _ip, _dp, _vp, _vp, _vp, _vp,
                                                   _pint]),
+    "aenet_structures_energy_and_forces_noalloc": (None, [C.c_int, _ip, _dp, _ip, _ip, _dp, _vp, _vp, _vp, _vp,
+                                                          _pint]),
     "aenet_amd_batch_create": (None, [C.c_int, _ip, _dp, _ip, _dp, C.POINTER(_vp), _pint]),
     "aenet_amd_batch_destroy": (None, [_vp]),
     "aenet_amd_batch_natoms": (C.c_int, [_vp]),

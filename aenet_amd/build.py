@@ -20,7 +20,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("AENET_AMD_ARCH", "gfx950")
 
-DEVICE_SOURCES = ["kernels.hip", "atom_cheb.hip", "atom_bp.hip", "train.hip", "mc.hip"]
+DEVICE_SOURCES = ["kernels.hip", "atom_cheb.hip", "atom_bp.hip", "train.hip", "mc.hip", "noalloc.hip"]
 HOST_SOURCES = ["model.cpp", "plan.cpp", "engine.cpp", "mc.cpp", "capi.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + CSRC, "-I" + os.path.join(os.path.dirname(HERE), "include")]

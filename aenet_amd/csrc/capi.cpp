@@ -652,17 +652,48 @@ void aenet_nbl_neighbors(int iatom, int *nnb, double nbcoo[], double nbdist[], i
 void aenet_structures_energy_and_forces(int nstruct, const int natoms[], const double latt[], const int pbc[],
                                         const int types[], const double coords[], double E_tot[],
                                         double E_coh[], double E_atoms[], double F[], int *stat) {
+  aenet_amd::structures_energy_and_forces(nstruct, natoms, latt, pbc, types, coords, E_tot, E_coh, E_atoms, F,
+                                          nullptr, "aenet_structures_energy_and_forces", stat);
+}
+
+}  // extern "C"
+
+namespace aenet_amd {
+
+int set_last_error(int st, const std::string &msg) { return set_err(st, msg); }
+
+void structures_energy_and_forces(int nstruct, const int natoms[], const double latt[], const int pbc[],
+                                  const int types[], const double coords[], double E_tot[], double E_coh[],
+                                  double E_atoms[], double F[], ScratchArena *arena, const char *who, int *stat) {
   std::lock_guard<std::mutex> lk(g_mu);
   Global &g = G();
   if (!g.init || !g.set.loaded) {
     *stat = set_err(AENET_STATUS_ERR_INIT, "potentials not loaded");
     return;
   }
+  if (nstruct < 1 || !natoms || !latt || !pbc || !types || !coords) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, std::string(who) + ": no structures or missing arrays");
+    return;
+  }
+  hipStream_t s = lib_stream();
+  // with an arena, the previous call's work on it must be over before its
+  // memory is rewritten (an error path may have left kernels queued)
+  if (arena && hipStreamSynchronize(s) != hipSuccess) {
+    *stat = set_err(AENET_STATUS_ERR_DEVICE, "device error");
+    return;
+  }
+  ArenaScope scope(arena);
+  const auto fail = [&](int st, const std::string &msg) {
+    if (arena && arena->overflow)
+      return set_err(AENET_STATUS_ERR_CAPACITY, std::string(who) + ": the structures need more device scratch than its " +
+                                                    std::to_string(arena->cap >> 20) + " MiB");
+    return set_err(st, msg);
+  };
   std::string err;
   Batch b;
   int st = b.create(g.set, nstruct, natoms, latt, pbc, coords, err);
   if (st) {
-    *stat = set_err(st, err);
+    *stat = fail(st, err);
     return;
   }
   int N = b.natoms();
@@ -671,12 +702,11 @@ void aenet_structures_energy_and_forces(int nstruct, const int natoms[], const d
       *stat = set_err(AENET_STATUS_ERR_TYPE, "atom type out of range");
       return;
     }
-  hipStream_t s = lib_stream();
   DBuf<double> dc, dE, dES, dF;
   DBuf<int> dt;
   if (dc.upload(coords, 3 * (size_t)N, s) || dt.upload(types, N, s) || dE.alloc(N) || dES.alloc(2 * (size_t)nstruct) ||
       (F && dF.alloc(3 * (size_t)N))) {
-    *stat = set_err(AENET_STATUS_ERR_MALLOC, "device allocation failed");
+    *stat = fail(AENET_STATUS_ERR_MALLOC, "device allocation failed");
     return;
   }
   st = b.eval(*g.dev, dc.p, dt.p, dE.p, dES.p, F ? dF.p : nullptr, s);
@@ -689,7 +719,7 @@ void aenet_structures_energy_and_forces(int nstruct, const int natoms[], const d
     if (!st) st = b.check_flags(s, err);
   }
   if (st) {
-    *stat = set_err(st, err.empty() ? "kernel launch failed" : err);
+    *stat = fail(st, err.empty() ? "kernel launch failed" : err);
     return;
   }
   std::vector<double> es(2 * (size_t)nstruct);
@@ -706,6 +736,10 @@ void aenet_structures_energy_and_forces(int nstruct, const int natoms[], const d
   }
   *stat = AENET_STATUS_OK;
 }
+
+}  // namespace aenet_amd
+
+extern "C" {
 
 void aenet_amd_batch_create(int nstruct, const int natoms[], const double latt[], const int pbc[],
                             const double coords_host[], aenet_amd_batch **batch, int *stat) {

@@ -10,6 +10,11 @@
 
 namespace aenet_amd {
 
+ScratchArena *&current_arena() {
+  static thread_local ScratchArena *a = nullptr;
+  return a;
+}
+
 int DeviceModel::upload(const PotSet &ps, hipStream_t s) {
   ntypes = ps.ntypes;
   desc = ps.desc;

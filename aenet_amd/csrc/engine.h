@@ -14,11 +14,40 @@
 
 namespace aenet_amd {
 
+// A fixed region of device memory handed out front to back.  While a
+// thread has one installed (ArenaScope), every DBuf it allocates takes its
+// memory from the region instead of hipMalloc, and releasing it frees
+// nothing.  The region is the caller's; its contents are not cleared.
+struct ScratchArena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  bool overflow = false;  // a request did not fit
+  void *take(size_t bytes) {
+    const size_t b = (bytes + 255) & ~(size_t)255;
+    if (overflow || b > cap - used) {
+      overflow = true;
+      return nullptr;
+    }
+    void *q = base + used;
+    used += b;
+    return q;
+  }
+};
+ScratchArena *&current_arena();  // of the calling thread (null: hipMalloc)
+struct ArenaScope {
+  ScratchArena *prev;
+  explicit ArenaScope(ScratchArena *a) : prev(current_arena()) { current_arena() = a; }
+  ~ArenaScope() { current_arena() = prev; }
+  ArenaScope(const ArenaScope &) = delete;
+  ArenaScope &operator=(const ArenaScope &) = delete;
+};
+
 // RAII device buffer
 template <class T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0;
+  bool owned = false;  // from hipMalloc (not from an arena)
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
@@ -26,10 +55,17 @@ struct DBuf {
   int alloc(size_t count) {
     release();
     if (count == 0) count = 1;
+    if (ScratchArena *a = current_arena()) {
+      p = static_cast<T *>(a->take(count * sizeof(T)));
+      if (!p) return -1;
+      n = count;
+      return 0;
+    }
     if (hipMalloc((void **)&p, count * sizeof(T)) != hipSuccess) {
       p = nullptr;
       return -1;
     }
+    owned = true;
     n = count;
     return 0;
   }
@@ -39,9 +75,10 @@ struct DBuf {
     return 0;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && owned) (void)hipFree(p);
     p = nullptr;
     n = 0;
+    owned = false;
   }
 };
 
@@ -163,5 +200,15 @@ struct EnvScratch {
            const double *h_cj, const int *h_tj, bool forces, double *h_E, double *h_dFi, double *h_dFj,
            double *h_G, int G_stride, hipStream_t stream, std::string &err);
 };
+
+// Body of aenet_structures_energy_and_forces (capi.cpp).  With an arena,
+// every device buffer of the call (inputs, outputs, plan, cell lists,
+// neighbor lists) comes from it; a request beyond it fails with
+// AENET_STATUS_ERR_CAPACITY.
+void structures_energy_and_forces(int nstruct, const int natoms[], const double latt[], const int pbc[],
+                                  const int types[], const double coords[], double E_tot[], double E_coh[],
+                                  double E_atoms[], double F[], ScratchArena *arena, const char *who, int *stat);
+// records the calling thread's last error (aenet_amd_last_error); returns st
+int set_last_error(int st, const std::string &msg);
 
 }  // namespace aenet_amd

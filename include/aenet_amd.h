@@ -38,11 +38,30 @@ void aenet_structures_energy_and_forces(int nstruct, const int natoms[],
                                         double E_tot[], double E_coh[],
                                         double E_atoms[], double F[], int *stat);
 
+/* aenet_structures_energy_and_forces without device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results; every device buffer of the call (inputs,
+ * outputs, plan, cell and neighbor lists) is taken from one fixed 64 MiB
+ * device array that each GPU the library runs on holds until the process
+ * ends.  Structures that need more than that (about 20,000 atoms of dense
+ * bulk at a 6.5 A cutoff) are refused with AENET_AMD_ERR_CAPACITY and a
+ * message.  Calls are serialized by the library's lock; each call
+ * overwrites the array, so calls that overlapped on one GPU (from
+ * different threads of the library without that lock) would race.  The
+ * host-side planning still uses ordinary host memory. */
+void aenet_structures_energy_and_forces_noalloc(int nstruct, const int natoms[],
+                                                const double latt[], const int pbc[],
+                                                const int types[], const double coords[],
+                                                double E_tot[], double E_coh[],
+                                                double E_atoms[], double F[], int *stat);
+
 /* Device-resident batches: the structures' shapes (lattices, atom counts,
  * pbc) are fixed at creation; coordinates and types are device pointers
  * passed to each evaluation, so a caller that keeps its structures in HBM
  * (Monte Carlo, molecular dynamics, training) never copies them. */
 typedef struct aenet_amd_batch aenet_amd_batch;
+typedef struct aenet_amd_model aenet_amd_model;  /* potential-set handles, below */
 
 void aenet_amd_batch_create(int nstruct, const int natoms[], const double latt[],
                             const int pbc[], const double coords_host[],
@@ -121,7 +140,6 @@ void aenet_amd_batch_descriptors(aenet_amd_batch *batch, const double *d_coords,
  * handles.  Passing NULL as the model of an entry point below means the
  * set loaded with aenet_load_potential.
  * ------------------------------------------------------------------ */
-typedef struct aenet_amd_model aenet_amd_model;
 
 /* files[t] is the .nn potential of species atom_types[t] (type id t+1). */
 void aenet_amd_model_load(int ntypes, char *atom_types[], char *files[],

@@ -32,6 +32,15 @@ def test_library_exports_declared_symbols():
     assert set(funcs) | set(data) >= decl
 
 
+def test_headers_compile_as_c99(tmp_path):
+    """include/*.h are C headers: a C99 translation unit includes both."""
+    import subprocess
+    src = tmp_path / "h.c"
+    src.write_text('#include "aenet.h"\n#include "aenet_amd.h"\nint main(void) { return 0; }\n')
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"), "-c", str(src),
+                    "-o", str(tmp_path / "h.o")], check=True)
+
+
 def test_status_constants():
     L = _native.lib()
     vals = {n: C.c_int.in_dll(L, n).value for n in
