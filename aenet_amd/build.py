@@ -2,6 +2,9 @@
 
     python -m aenet_amd.build        # or: python aenet_amd/build.py
 
+It also builds aenet_amd/_lib/predict.x, the predict.x-compatible
+command-line driver (csrc/predict.cpp).
+
 Host C++ is compiled with -ffp-contract=off so that the FP64 plan
 arithmetic follows the reference's evaluation order; device code keeps
 contraction for the FP32 descriptor/network math (the FP64 neighbor test
@@ -73,6 +76,12 @@ def build(verbose=True, jobs=8):
     _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp, *objs,
           "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")], verbose)
     os.replace(tmp, LIB)
+    # predict.x: the batched command-line driver, linked against the library
+    exe = os.path.join(OUT_DIR, "predict.x")
+    _run([CXX, "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(os.path.dirname(HERE), "include"),
+          os.path.join(CSRC, "predict.cpp"), "-o", exe + ".tmp", "-L" + OUT_DIR, "-laenet_amd",
+          "-Wl,-rpath,$ORIGIN"], verbose)
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 
