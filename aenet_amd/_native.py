@@ -60,6 +60,7 @@ SIGNATURES = {
     "aenet_amd_batch_check": (None, [_vp, _vp, _pint]),
     "aenet_amd_batch_set_timing": (None, [_vp, C.c_int]),
     "aenet_amd_batch_kernel_time": (None, [_vp, C.POINTER(C.c_double), _pint, _pint]),
+    "aenet_amd_batch_search_time": (None, [_vp, C.POINTER(C.c_double), _pint, _pint]),
     "aenet_amd_batch_neighbors": (None, [_vp, _vp, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
     "aenet_amd_batch_descriptors": (None, [_vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _pint]),
     "aenet_amd_model_load": (None, [C.c_int, C.POINTER(C.c_char_p), C.POINTER(C.c_char_p), C.POINTER(_vp),

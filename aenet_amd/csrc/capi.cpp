@@ -810,6 +810,15 @@ void aenet_amd_batch_kernel_time(aenet_amd_batch *batch, double *ms, int *count,
   if (*stat) set_err(*stat, "event timing failed");
 }
 
+void aenet_amd_batch_search_time(aenet_amd_batch *batch, double *ms, int *count, int *stat) {
+  if (!batch) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch");
+    return;
+  }
+  *stat = batch->batch.search_time(*ms, *count);
+  if (*stat) set_err(*stat, "event timing failed");
+}
+
 void aenet_amd_batch_check(aenet_amd_batch *batch, void *stream, int *stat) {
   if (!batch) {
     *stat = set_err(AENET_STATUS_ERR_INIT, "no batch");

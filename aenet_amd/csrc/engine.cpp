@@ -285,7 +285,17 @@ int Batch::eval(const DeviceModel &dm, const double *d_coords, const int *d_type
   ModelDev md = dm.view();
   NbDev nb = nb_view();
   launch_binning(b, cell_tmp_.p, d_F, stream);
-  launch_search(b, nb, stream);
+  {
+    hipEvent_t s0 = nullptr, s1 = nullptr;
+    if (timing_ && hipEventCreate(&s0) == hipSuccess && hipEventCreate(&s1) == hipSuccess)
+      (void)hipEventRecord(s0, stream);
+    launch_search(b, nb, stream);
+    if (s0 && s1) {
+      (void)hipEventRecord(s1, stream);
+      sev_.push_back(s0);
+      sev_.push_back(s1);
+    }
+  }
   OutDev o;
   std::memset(&o, 0, sizeof(o));
   o.E_atom = d_E_atom ? d_E_atom : scratch_E_.p;
@@ -480,6 +490,11 @@ int Batch::autotune(const DeviceModel &dm, const double *d_coords, const int *d_
     }
   }
   timing_ = was_timing;
+  {
+    double t1 = 0.0;
+    int c1 = 0;
+    (void)search_time(t1, c1);  // the tuner's own search launches are not reported
+  }
   if (st) {
     tune_.erase(&dm);
     err = "evaluation failed while tuning";
@@ -501,24 +516,29 @@ int Batch::autotune(const DeviceModel &dm, const double *d_coords, const int *d_
 
 void Batch::set_timing(bool on) { timing_ = on; }
 
-int Batch::kernel_time(double &ms, int &count) {
+static int drain_events(std::vector<hipEvent_t> &ev, double &ms, int &count) {
   ms = 0.0;
   count = 0;
   int st = AENET_STATUS_OK;
-  for (size_t i = 0; i + 1 < ev_.size(); i += 2) {
+  for (size_t i = 0; i + 1 < ev.size(); i += 2) {
     float t = 0.0f;
-    if (hipEventSynchronize(ev_[i + 1]) != hipSuccess || hipEventElapsedTime(&t, ev_[i], ev_[i + 1]) != hipSuccess)
+    if (hipEventSynchronize(ev[i + 1]) != hipSuccess || hipEventElapsedTime(&t, ev[i], ev[i + 1]) != hipSuccess)
       st = AENET_STATUS_ERR_DEVICE;
     ms += t;
     ++count;
   }
-  for (auto e : ev_) (void)hipEventDestroy(e);
-  ev_.clear();
+  for (auto e : ev) (void)hipEventDestroy(e);
+  ev.clear();
   return st;
 }
 
+int Batch::kernel_time(double &ms, int &count) { return drain_events(ev_, ms, count); }
+
+int Batch::search_time(double &ms, int &count) { return drain_events(sev_, ms, count); }
+
 Batch::~Batch() {
   for (auto e : ev_) (void)hipEventDestroy(e);
+  for (auto e : sev_) (void)hipEventDestroy(e);
 }
 
 static std::string flag_text(int f) {

@@ -147,6 +147,8 @@ class Batch {
   // summed duration (ms) and count of the timed atom-kernel launches since
   // the last call; synchronizes on the recorded events
   int kernel_time(double &ms, int &count);
+  // the same for the neighbor-search launches of full evaluations
+  int search_time(double &ms, int &count);
   ~Batch();
   int natoms() const { return plan_.natoms; }
   int nb_cap() const { return nb_cap_; }
@@ -186,7 +188,7 @@ class Batch {
   DBuf<float> nb_R_;
   NbDev nb_view() const;
   bool timing_ = false;
-  std::vector<hipEvent_t> ev_;
+  std::vector<hipEvent_t> ev_, sev_;
   DBuf<double> scratch_E_;
   DBuf<double> scratch_G_;
 };

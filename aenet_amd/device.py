@@ -154,6 +154,15 @@ class DeviceBatch:
         check_status(stat.value, "aenet_amd_batch_kernel_time")
         return ms.value, n.value
 
+    def search_time(self):
+        """(summed ms, launches) of the neighbor search since the last call."""
+        ms = C.c_double(0.0)
+        n = C.c_int(0)
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_search_time(self._h, C.byref(ms), C.byref(n), C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_search_time")
+        return ms.value, n.value
+
     def neighbors(self, cap, coords=None, types=None):
         coords = self.coords if coords is None else coords
         types = self.types if types is None else types

@@ -105,6 +105,9 @@ def cpu_baseline(procs, stride, reps):
         "end_to_end": atoms / t_e2e,
         "cpu_seconds": cpu_s,
         "wall_seconds": wall,
+        "host_cores_total": os.cpu_count(),
+        "host_cores_affinity": sys_cores(),
+        "cores_note": "worker processes capped at 16, the host CPU share of one GPU on these machines",
     }
 
 
@@ -254,6 +257,49 @@ def mc_throughput(steps, burn):
     return out
 
 
+def per_atom_abi(ann, db, cpu, natoms=400):
+    """The per-atom drop-in (aenet_atomic_energy_and_forces, one call per
+    atom with its neighbor list from host memory, as predict.x and the ASE
+    calculator call it) on the first atoms of the alloy, against the
+    reference library's per-atom rate on one core (its kernel-equivalent
+    rate over the worker processes, divided by their number)."""
+    import numpy as np
+    nb = db.neighbors(256)
+    nnb = nb["nnb"].cpu().numpy()
+    coo = nb["coo"].cpu().numpy()
+    lst = nb["list"].cpu().numpy()
+    typ = nb["type"].cpu().numpy()
+    x = db.coords.view(-1, 3).cpu().numpy()
+    t_at = db.types.cpu().numpy()
+    names = ["A", "B"]
+    F = np.zeros((db.natoms, 3))
+    # lattice-wrapped centre: the lists hold the image coordinates
+    L = np.asarray(synth_cell(), dtype=np.float64)
+    frac = x @ np.linalg.inv(L)
+    ci = (frac - np.floor(frac)) @ L
+    for i in range(20):   # warm-up
+        k = nnb[i]
+        ann.atomic_energy_and_forces(ci[i], names[t_at[i] - 1], i + 1, coo[i, :k],
+                                     [names[t - 1] for t in typ[i, :k]], lst[i, :k], F)
+    t0 = time.perf_counter()
+    for i in range(natoms):
+        k = nnb[i]
+        ann.atomic_energy_and_forces(ci[i], names[t_at[i] - 1], i + 1, coo[i, :k],
+                                     [names[t - 1] for t in typ[i, :k]], lst[i, :k], F)
+    dt = time.perf_counter() - t0
+    out = {"workload": "aenet_atomic_energy_and_forces once per atom, %d atoms of alloy4000, neighbor lists "
+                       "in host memory (one launch and two copies per call)" % natoms,
+           "gpu_atoms_per_s": natoms / dt, "us_per_call": dt / natoms * 1e6}
+    if cpu and cpu.get("value"):
+        out["reference_atoms_per_s_one_core"] = cpu["value"] / cpu["cores"]
+    return out
+
+
+def synth_cell():
+    from aenet_amd import synth
+    return synth.c2_alloy4000().cell
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,6 +313,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched Behler-Parrinello measurement")
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo measurement")
+    ap.add_argument("--no-per-atom", action="store_true", help="skip the per-atom C-ABI measurement")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the occupancy model's launch shape instead of the measured one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -290,6 +337,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
+        # at most 16 workers: the CPU share of one GPU on these machines
         procs = args.cpu_procs or min(16, sys_cores())
         try:
             cpu = cpu_baseline(procs, args.cpu_stride, args.cpu_passes)
@@ -317,6 +365,7 @@ def main():
         tune = {"chosen": {"waves_per_workgroup": chosen[0], "weights_in_lds": chosen[1]},
                 "candidates_us": [{"waves_per_workgroup": w, "weights_in_lds": s, "kernel_us": t * 1e3}
                                   for (w, s, t) in rows]}
+    # eager launches (five kernels per step)
     for _ in range(args.warmup):
         db.eval(check=False)
     db.check()
@@ -327,8 +376,35 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     db.check()
-    ms = (t1 - t0) * 1e3 / args.steps
+    eager_ms = (t1 - t0) * 1e3 / args.steps
+    E_eager = db.E_atoms.clone()
+    # the headline: one step captured in a HIP graph and replayed (the
+    # step allocates and clears nothing, so it captures whole)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(2):
+            db.eval(check=False)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        db.eval(check=False)
+    for _ in range(args.warmup):
+        graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    db.check()
+    graph_ms = (t1 - t0) * 1e3 / args.steps
+    # the headline is the faster of the two ways of issuing the same step
+    ms = min(graph_ms, eager_ms)
     value = natoms / (ms * 1e-3)
+    if not torch.equal(db.E_atoms, E_eager):
+        raise RuntimeError("graph replay and eager evaluation differ")
     E_tot = float(db.E_struct[0].item())
 
     # roofline of the fused atom kernel: HIP events around its launches
@@ -337,9 +413,16 @@ def main():
     for _ in range(nt):
         db.eval(check=False)
     kms, kcount = db.kernel_time()
+    sms, scount = db.search_time()
     db.set_timing(False)
     k_avg_s = kms / kcount * 1e-3
+    s_avg_s = sms / scount * 1e-3
     n_p, n_t = pair_triplet_counts(db)
+    entries = int(db.neighbors(256)["nnb"].sum().item())
+    # neighbor search: algorithmic bytes = the list it writes (16 B of
+    # float4 R + 4 B of index per entry) + 4 B of count per atom + the
+    # coordinates and types it reads once (28 B per atom)
+    search_bytes = 20 * entries + 32 * natoms
     flops = n_p * FLOP_PAIR + n_t * FLOP_TRIPLET + natoms * FLOP_ATOM
     achieved = flops / k_avg_s
     traffic = None
@@ -370,8 +453,8 @@ def main():
             "E_tot_eV": E_tot,
         },
         "roofline": {
-            "bound": "mfma",
-            "pipe": "FP32 VALU (MI355X FP32 vector and FP32 matrix peaks are both 157.3 TFLOP/s)",
+            "bound": "valu",
+            "pipe": "FP32 VALU: the atom kernel issues no MFMA; peak = MI355X FP32 vector 157.3 TFLOP/s",
             "kernel": "k_atom_cheb<19,7>",
             "achieved": achieved / 1e12,
             "peak": FP32_PEAK / 1e12,
@@ -386,11 +469,30 @@ def main():
             "triplets": n_t,
             "launch_shape": tune,
         },
+        "step": {
+            "timing": "the step (binning, search, atom kernel, structure sums: five launches) issued one by "
+                      "one (eager) and as one captured HIP graph replayed (graph); value and ms_per_step "
+                      "use the faster, here %s" % ("graph" if graph_ms <= eager_ms else "eager"),
+            "graph_ms_per_step": graph_ms,
+            "eager_ms_per_step": eager_ms,
+        },
+        "neighbor_search": {
+            "kernel": "k_search_cells",
+            "bound": "hbm",
+            "kernel_ms": s_avg_s * 1e3,
+            "list_entries": entries,
+            "bytes_per_launch": search_bytes,
+            "achieved_GBps": search_bytes / s_avg_s / 1e9,
+            "peak_GBps": HBM_PEAK / 1e9,
+            "frac": search_bytes / s_avg_s / HBM_PEAK,
+        },
         "cpu_baseline": cpu,
     }
     if cpu and cpu.get("value"):
         out["gpu_over_cpu"] = value / cpu["value"]
         out["gpu_over_cpu_end_to_end"] = value / cpu["end_to_end"]
+    if not args.no_per_atom:
+        out["per_atom_abi"] = per_atom_abi(ann, db, cpu)
     ann.release()
     if not args.no_batch:
         out["batched_bp"] = batched_bp(max(5, min(args.steps, 20)), 2)

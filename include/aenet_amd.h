@@ -100,6 +100,10 @@ void aenet_amd_batch_check(aenet_amd_batch *batch, void *stream, int *stat);
 void aenet_amd_batch_set_timing(aenet_amd_batch *batch, int enable);
 void aenet_amd_batch_kernel_time(aenet_amd_batch *batch, double *ms, int *count,
                                  int *stat);
+/* the same for the neighbor-search launches (k_search_cells) of the timed
+ * evaluations */
+void aenet_amd_batch_search_time(aenet_amd_batch *batch, double *ms, int *count,
+                                 int *stat);
 
 /* Autotuner of the atom kernel's launch shape for this batch and model
  * (NULL: the potentials of aenet_load_potential): every feasible pair
