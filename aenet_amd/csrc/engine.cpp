@@ -569,10 +569,14 @@ int Batch::check_flags(hipStream_t stream, std::string &err) {
   return AENET_STATUS_OK;
 }
 
+EnvScratch::~EnvScratch() {
+  if (pinned) (void)hipHostFree(pinned);
+}
+
 int EnvScratch::eval(const DeviceModel &dm, int nenv, const int *h_start, const int *h_ctype, const double *h_ci,
                      const double *h_cj, const int *h_tj, bool forces, double *h_E, double *h_dFi,
                      double *h_dFj, double *h_G, int G_stride, hipStream_t stream, std::string &err) {
-  size_t nnb = (size_t)h_start[nenv];
+  const size_t nnb = (size_t)h_start[nenv];
   int max_n = 1;
   for (int i = 0; i < nenv; ++i) max_n = std::max(max_n, h_start[i + 1] - h_start[i]);
   if (max_n > AENET_AMD_NB_CAP) {
@@ -580,39 +584,43 @@ int EnvScratch::eval(const DeviceModel &dm, int nenv, const int *h_start, const 
     return AENET_STATUS_ERR_CAPACITY;
   }
   max_n = (max_n + 31) / 32 * 32;
-  bool bad = false;
-  if ((size_t)nenv > cap_env || !start.p) {
-    cap_env = std::max((size_t)nenv, (size_t)64);
-    bad |= start.alloc(cap_env + 1) != 0;
-    bad |= ctype.alloc(cap_env) != 0;
-    bad |= ci.alloc(3 * cap_env) != 0;
-    bad |= E.alloc(cap_env) != 0;
-    bad |= dFi.alloc(3 * cap_env) != 0;
-    bad |= flags.alloc(1) != 0;
-  }
-  if (nnb > cap_nb || !cj.p) {
-    cap_nb = std::max(nnb, (size_t)1024);
-    bad |= cj.alloc(3 * cap_nb) != 0;
-    bad |= tj.alloc(cap_nb) != 0;
-    bad |= dFj.alloc(3 * cap_nb) != 0;
+  // packed layout (byte offsets, 8-aligned double sections)
+  auto up8 = [](size_t v) { return (v + 7) / 8 * 8; };
+  const size_t o_start = 0, o_ctype = o_start + 4 * (size_t)(nenv + 1), o_tj = o_ctype + 4 * (size_t)nenv;
+  const size_t o_ci = up8(o_tj + 4 * nnb), o_cj = o_ci + 24 * (size_t)nenv, o_flags = o_cj + 24 * nnb;
+  const size_t o_E = o_flags + 8, o_dFi = o_E + 8 * (size_t)nenv, o_dFj = o_dFi + 24 * (size_t)nenv;
+  const size_t o_end = o_dFj + 24 * nnb;
+  if (o_end > bytes || !dev.p || !pinned) {
+    const size_t want = std::max(o_end + o_end / 2, (size_t)(64 << 10));
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+    bytes = 0;
+    if (dev.alloc(want) || hipHostMalloc((void **)&pinned, want, hipHostMallocDefault) != hipSuccess) {
+      pinned = nullptr;
+      err = "device or pinned host allocation failed";
+      return AENET_STATUS_ERR_MALLOC;
+    }
+    bytes = want;
   }
   DBuf<double> G;
-  if (h_G) bad |= G.alloc((size_t)nenv * G_stride) != 0;
-  if (bad) {
+  if (h_G && G.alloc((size_t)nenv * G_stride)) {
     err = "device allocation failed";
     return AENET_STATUS_ERR_MALLOC;
   }
-  (void)hipMemcpyAsync(start.p, h_start, (nenv + 1) * sizeof(int), hipMemcpyHostToDevice, stream);
-  (void)hipMemcpyAsync(ctype.p, h_ctype, nenv * sizeof(int), hipMemcpyHostToDevice, stream);
-  (void)hipMemcpyAsync(ci.p, h_ci, 3 * nenv * sizeof(double), hipMemcpyHostToDevice, stream);
-  if (nnb) {
-    (void)hipMemcpyAsync(cj.p, h_cj, 3 * nnb * sizeof(double), hipMemcpyHostToDevice, stream);
-    (void)hipMemcpyAsync(tj.p, h_tj, nnb * sizeof(int), hipMemcpyHostToDevice, stream);
+  std::memcpy(pinned + o_start, h_start, 4 * (size_t)(nenv + 1));
+  std::memcpy(pinned + o_ctype, h_ctype, 4 * (size_t)nenv);
+  if (nnb) std::memcpy(pinned + o_tj, h_tj, 4 * nnb);
+  std::memcpy(pinned + o_ci, h_ci, 24 * (size_t)nenv);
+  if (nnb) std::memcpy(pinned + o_cj, h_cj, 24 * nnb);
+  std::memset(pinned + o_flags, 0, 8);
+  if (hipMemcpyAsync(dev.p, pinned, o_E, hipMemcpyHostToDevice, stream) != hipSuccess) {
+    err = "device error";
+    return AENET_STATUS_ERR_DEVICE;
   }
-  launch_fill_i32(flags.p, 0, 1, stream);
+  char *d = dev.p;
   BatchDev b;
   std::memset(&b, 0, sizeof(b));
-  b.flags = flags.p;
+  b.flags = reinterpret_cast<int *>(d + o_flags);
   OutDev o;
   std::memset(&o, 0, sizeof(o));
   o.G = h_G ? G.p : nullptr;
@@ -621,14 +629,14 @@ int EnvScratch::eval(const DeviceModel &dm, int nenv, const int *h_start, const 
   o.desc_only = h_G && !h_E;
   EnvDev e;
   e.nenv = nenv;
-  e.env_start = start.p;
-  e.env_type = ctype.p;
-  e.coo_i = ci.p;
-  e.coo_j = cj.p;
-  e.type_j = tj.p;
-  e.E = E.p;
-  e.dF_i = dFi.p;
-  e.dF_j = dFj.p;
+  e.env_start = reinterpret_cast<int *>(d + o_start);
+  e.env_type = reinterpret_cast<int *>(d + o_ctype);
+  e.coo_i = reinterpret_cast<double *>(d + o_ci);
+  e.coo_j = reinterpret_cast<double *>(d + o_cj);
+  e.type_j = reinterpret_cast<int *>(d + o_tj);
+  e.E = reinterpret_cast<double *>(d + o_E);
+  e.dF_i = reinterpret_cast<double *>(d + o_dFi);
+  e.dF_j = reinterpret_cast<double *>(d + o_dFj);
   ModelDev md = dm.view();
   NbDev nb;
   std::memset(&nb, 0, sizeof(nb));
@@ -640,21 +648,23 @@ int EnvScratch::eval(const DeviceModel &dm, int nenv, const int *h_start, const 
     err = "kernel launch failed";
     return AENET_STATUS_ERR_DEVICE;
   }
-  int f = 0;
-  (void)hipMemcpyAsync(&f, flags.p, sizeof(int), hipMemcpyDeviceToHost, stream);
-  if (h_E) (void)hipMemcpyAsync(h_E, E.p, nenv * sizeof(double), hipMemcpyDeviceToHost, stream);
-  if (forces) {
-    (void)hipMemcpyAsync(h_dFi, dFi.p, 3 * nenv * sizeof(double), hipMemcpyDeviceToHost, stream);
-    if (nnb) (void)hipMemcpyAsync(h_dFj, dFj.p, 3 * nnb * sizeof(double), hipMemcpyDeviceToHost, stream);
-  }
+  const size_t back = forces ? o_end : o_dFi;
+  (void)hipMemcpyAsync(pinned + o_flags, d + o_flags, back - o_flags, hipMemcpyDeviceToHost, stream);
   if (h_G) (void)hipMemcpyAsync(h_G, G.p, (size_t)nenv * G_stride * sizeof(double), hipMemcpyDeviceToHost, stream);
   if (hipStreamSynchronize(stream) != hipSuccess) {
     err = "device error";
     return AENET_STATUS_ERR_DEVICE;
   }
+  int f = 0;
+  std::memcpy(&f, pinned + o_flags, sizeof(int));
   if (f) {
     err = "kernel reported:" + flag_text(f);
     return (f & AENET_AMD_FLAG_BAD_TYPE) ? AENET_STATUS_ERR_TYPE : AENET_STATUS_ERR_CAPACITY;
+  }
+  if (h_E) std::memcpy(h_E, pinned + o_E, 8 * (size_t)nenv);
+  if (forces) {
+    std::memcpy(h_dFi, pinned + o_dFi, 24 * (size_t)nenv);
+    if (nnb) std::memcpy(h_dFj, pinned + o_dFj, 24 * nnb);
   }
   return AENET_STATUS_OK;
 }

@@ -193,11 +193,15 @@ class Batch {
   DBuf<double> scratch_G_;
 };
 
-// explicit environments (per-atom entry points)
+// explicit environments (per-atom entry points).  Inputs and outputs of a
+// call travel in one packed block each way between pinned host memory and
+// one device buffer: [start | ctype | tj | ci | cj | flags | E | dFi | dFj]
+// (the call's latency is mostly copies, so one of each).
 struct EnvScratch {
-  DBuf<int> start, ctype, tj, flags;
-  DBuf<double> ci, cj, E, dFi, dFj;
-  size_t cap_env = 0, cap_nb = 0;
+  DBuf<char> dev;
+  char *pinned = nullptr;
+  size_t bytes = 0;
+  ~EnvScratch();
   int eval(const DeviceModel &dm, int nenv, const int *h_start, const int *h_ctype, const double *h_ci,
            const double *h_cj, const int *h_tj, bool forces, double *h_E, double *h_dFi, double *h_dFj,
            double *h_G, int G_stride, hipStream_t stream, std::string &err);
