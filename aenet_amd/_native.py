@@ -43,6 +43,7 @@ SIGNATURES = {
     "aenet_sfb_final": (None, [_pint]),
     "aenet_sfb_nvalues": (C.c_int, []),
     "aenet_sfb_eval": (None, [C.c_int, _dp, C.c_int, _ip, _dp, C.c_int, _dp, _pint]),
+    "aenet_sfb_eval_deriv": (None, [C.c_int, _dp, C.c_int, _ip, _dp, C.c_int, _dp, _vp, _vp, _pint]),
     "aenet_sfb_reconstruct_radial": (None, [C.c_int, _dp, C.c_int, _dp, _dp, _pint]),
     "aenet_nbl_init": (None, [_dp, C.c_int, _ip, _dp, C.c_bool, C.c_bool]),
     "aenet_nbl_final": (None, []),

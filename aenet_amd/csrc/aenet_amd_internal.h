@@ -182,6 +182,8 @@ struct OutDev {
   int G_stride;
   int desc_only;       // stop after the descriptors
   const double *g_atom;  // optional [natoms]: forces of sum_i g_i E_i instead of sum_i E_i
+  int coef_unit;       // explicit environments: > 0 replaces the network's dE/dG of environment g
+                       // by the unit vector of feature g mod coef_unit (descriptor derivatives)
   // neighbor-list mode (bit-exact lists)
   int nb_cap;
   int *nb_count;

@@ -470,6 +470,61 @@ void aenet_sfb_eval(int itype0, double coo0[3], int nat, int itype1[], double co
   if (st) *stat = set_err(st, err);
 }
 
+// sfb_eval with its derivatives (deriv0, deriv1 of src/ext/sfbasis.f90:
+// 440-571, which aenet_sfb_eval does not return): the environment is
+// evaluated as nv copies, copy f with the unit vector of feature f in
+// place of the network's dE/dG, so the force kernel returns -dG_f/dR_j and
+// -dG_f/dR_0.  The derivatives are FP32 sums (relative error ~1e-6).
+void aenet_sfb_eval_deriv(int itype0, double coo0[3], int nat, int itype1[], double coo1[], int nv, double values[],
+                          double deriv0[], double deriv1[], int *stat) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  Global &g = G();
+  *stat = AENET_STATUS_OK;
+  if (!g.sfb_init) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "aenet_sfb_eval_deriv: not initialized");
+    return;
+  }
+  if (nv != g.sfb_N || nat < 0) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_sfb_eval_deriv: wrong number of basis functions");
+    return;
+  }
+  const int nt = g.sfb_set.ntypes;
+  if (itype0 < 1 || itype0 > nt) {
+    *stat = set_err(AENET_STATUS_ERR_TYPE, "aenet_sfb_eval_deriv: type out of range");
+    return;
+  }
+  for (int j = 0; j < nat; ++j)
+    if (itype1[j] < 1 || itype1[j] > nt) {
+      *stat = set_err(AENET_STATUS_ERR_TYPE, "aenet_sfb_eval_deriv: type out of range");
+      return;
+    }
+  std::vector<int> start(nv + 1), ct(nv, itype0), tj((size_t)nv * nat);
+  std::vector<double> ci(3 * (size_t)nv), cj(3 * (size_t)nv * nat), E(nv), dFi(3 * (size_t)nv),
+      dFj(3 * (size_t)nv * nat), G((size_t)nv * nv);
+  for (int f = 0; f <= nv; ++f) start[f] = f * nat;
+  for (int f = 0; f < nv; ++f) {
+    std::memcpy(&ci[3 * (size_t)f], coo0, 3 * sizeof(double));
+    std::memcpy(&cj[3 * (size_t)f * nat], coo1, 3 * (size_t)nat * sizeof(double));
+    std::memcpy(&tj[(size_t)f * nat], itype1, (size_t)nat * sizeof(int));
+  }
+  std::string err;
+  int st = g.env.eval(*g.sfb_dev, nv, start.data(), ct.data(), ci.data(), cj.data(), tj.data(), true, E.data(),
+                      dFi.data(), dFj.data(), G.data(), nv, lib_stream(), err, nv);
+  if (st) {
+    *stat = set_err(st, err);
+    return;
+  }
+  std::memcpy(values, G.data(), (size_t)nv * sizeof(double));
+  // deriv0(3, nv), deriv1(3, nv, nat), column-major
+  for (int f = 0; f < nv; ++f)
+    for (int x = 0; x < 3; ++x) {
+      if (deriv0) deriv0[3 * (size_t)f + x] = -dFi[3 * (size_t)f + x];
+      if (deriv1)
+        for (int j = 0; j < nat; ++j)
+          deriv1[x + 3 * ((size_t)f + (size_t)nv * j)] = -dFj[3 * ((size_t)f * nat + j) + x];
+    }
+}
+
 // sfb_reconstruct_radial (src/ext/sfbasis.f90:581-637): the radial
 // distribution function from the first r_N expansion coefficients
 void aenet_sfb_reconstruct_radial(int nv, double values[], int nx, double x[], double y[], int *stat) {

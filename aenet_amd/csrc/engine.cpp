@@ -575,7 +575,8 @@ EnvScratch::~EnvScratch() {
 
 int EnvScratch::eval(const DeviceModel &dm, int nenv, const int *h_start, const int *h_ctype, const double *h_ci,
                      const double *h_cj, const int *h_tj, bool forces, double *h_E, double *h_dFi,
-                     double *h_dFj, double *h_G, int G_stride, hipStream_t stream, std::string &err) {
+                     double *h_dFj, double *h_G, int G_stride, hipStream_t stream, std::string &err,
+                     int coef_unit) {
   const size_t nnb = (size_t)h_start[nenv];
   int max_n = 1;
   for (int i = 0; i < nenv; ++i) max_n = std::max(max_n, h_start[i + 1] - h_start[i]);
@@ -627,6 +628,7 @@ int EnvScratch::eval(const DeviceModel &dm, int nenv, const int *h_start, const 
   o.G_scaled = 0;
   o.G_stride = G_stride;
   o.desc_only = h_G && !h_E;
+  o.coef_unit = coef_unit;
   EnvDev e;
   e.nenv = nenv;
   e.env_start = reinterpret_cast<int *>(d + o_start);

@@ -202,9 +202,11 @@ struct EnvScratch {
   char *pinned = nullptr;
   size_t bytes = 0;
   ~EnvScratch();
+  // coef_unit > 0: environment g differentiates descriptor feature
+  // g mod coef_unit instead of the energy (h_dFi, h_dFj = -dG/dR)
   int eval(const DeviceModel &dm, int nenv, const int *h_start, const int *h_ctype, const double *h_ci,
            const double *h_cj, const int *h_tj, bool forces, double *h_E, double *h_dFi, double *h_dFj,
-           double *h_G, int G_stride, hipStream_t stream, std::string &err);
+           double *h_G, int G_stride, hipStream_t stream, std::string &err, int coef_unit = 0);
 };
 
 // Body of aenet_structures_energy_and_forces (capi.cpp).  With an arena,

@@ -62,6 +62,14 @@ int aenet_sfb_nvalues(void);
 void aenet_sfb_eval(int itype0, double coo0[3], int nat, int itype1[],
                     double coo1[], int nv, double values[], int *stat);
 /* src/aenet.h:36-37, src/aenet.f90:775-799 */
+/* Not in the reference's header: sfb_eval with the derivatives it computes
+ * internally (src/ext/sfbasis.f90:440-571) -- deriv0(3,nv) with respect to
+ * the central atom and deriv1(3,nv,nat) with respect to each neighbor,
+ * column-major as there (either may be NULL).  Values are unscaled, as in
+ * aenet_sfb_eval. */
+void aenet_sfb_eval_deriv(int itype0, double coo0[3], int nat, int itype1[],
+                          double coo1[], int nv, double values[], double deriv0[],
+                          double deriv1[], int *stat);
 void aenet_sfb_reconstruct_radial(int nv, double values[], int nx,
                                   double x[], double y[], int *stat);
 

@@ -347,3 +347,45 @@ def test_nbl_neighbors_small_buffer_is_refused(device):
         L.aenet_nbl_final()
     finally:
         ann.release()
+
+
+def test_sfb_eval_deriv_matches_oracle(device):
+    """aenet_sfb_eval_deriv: the values of aenet_sfb_eval and the
+    derivatives sfb_eval computes (src/ext/sfbasis.f90:440-571) with
+    respect to the central atom and to every neighbor, against the FP64
+    oracle's unscaled descriptor derivatives (pinned to the reference by
+    bit-identical forces and by central differences, tests/test_oracle.py).
+    The derivatives are FP32 sums: within 2e-5 absolute plus 1e-5 relative."""
+    import ctypes as C
+    from aenet_amd import _native
+    from oracle_lib import Oracle
+    L = _native.lib()
+    stat = C.c_int(0)
+    L.aenet_sfb_init(2, (C.c_char_p * 2)(b"A", b"B"), 18, 6, 6.5, 4.5, C.byref(stat))
+    assert stat.value == 0, _native.last_error()
+    try:
+        nv = L.aenet_sfb_nvalues()
+        rng = np.random.Generator(np.random.PCG64(5))
+        coo = rng.uniform(-4, 4, size=(36, 3))
+        coo = coo[np.linalg.norm(coo, axis=1) > 1.5]
+        t = rng.integers(1, 3, len(coo)).astype(np.int32)
+        c0 = np.array([0.1, -0.2, 0.05])
+        vals = np.zeros(nv)
+        d0 = np.zeros(3 * nv)
+        d1 = np.zeros(3 * nv * len(t))
+        L.aenet_sfb_eval_deriv(1, c0, len(t), t, np.ascontiguousarray(coo).reshape(-1), nv, vals, d0.ctypes.data,
+                               d1.ctypes.data, C.byref(stat))
+        assert stat.value == 0, _native.last_error()
+        o = Oracle(["A", "B"], pot_files("alloy", ["A", "B"]))
+        G, dGi, dGj = o.descriptors(1, c0, coo, t, scaled=False, derivs=True)
+        deriv0 = d0.reshape(nv, 3)                     # (3, nv) column-major
+        deriv1 = d1.reshape(len(t), nv, 3)             # (3, nv, nat) column-major
+        assert np.allclose(vals, G, rtol=2e-6, atol=2e-5)
+        e0 = np.abs(deriv0 - dGi).max()
+        e1 = np.abs(deriv1 - dGj).max()
+        print("max |d deriv0| %.3e, |d deriv1| %.3e (|dG/dR| up to %.2f)" % (e0, e1, np.abs(dGj).max()))
+        assert np.allclose(deriv0, dGi, rtol=1e-5, atol=2e-5)
+        assert np.allclose(deriv1, dGj, rtol=1e-5, atol=2e-5)
+        assert np.abs(dGj).max() > 0.1
+    finally:
+        L.aenet_sfb_final(C.byref(stat))
