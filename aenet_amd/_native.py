@@ -70,6 +70,8 @@ SIGNATURES = {
     "aenet_amd_model_info": (None, [_vp, C.POINTER(C.c_double), _pint, _pint, _pint]),
     "aenet_amd_model_species_weights": (None, [_vp, C.c_int, _pint, _pint, _pint]),
     "aenet_amd_model_weights": (None, [_vp, _vp, C.c_int, _vp, _pint]),
+    "aenet_amd_model_descriptor_derivatives": (None, [_vp, C.c_int, _ip, _ip, _dp, _dp, _ip, C.c_int, _vp, _vp,
+                                                      _vp, _pint]),
     "aenet_amd_batch_create_model": (None, [_vp, C.c_int, _ip, _dp, _ip, _dp, C.POINTER(_vp), _pint]),
     "aenet_amd_batch_eval_model": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
     "aenet_amd_batch_autotune": (None, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _pint, _pint, _vp, _pint]),

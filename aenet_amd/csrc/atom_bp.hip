@@ -596,8 +596,17 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
     if (o.desc_only) continue;
 
     // ------------------------------------------------------------ 4
-    const float yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES)
-                                  : network(md, sp, (global_weights)md.W, S.net, FORCES);
+    float yout;
+    if (MODE == 1 && o.coef_unit > 0) {
+      // derivatives of one descriptor feature instead of the energy
+      const int fu = g % o.coef_unit;
+      for (int f = lane; f < sp.nsf; f += WAVE) S.net.coef[f] = f == fu ? 1.0f : 0.0f;
+      wave_sync();
+      yout = 0.0f;
+    } else {
+      yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES)
+                        : network(md, sp, (global_weights)md.W, S.net, FORCES);
+    }
     if (lane == 0) {
       if (MODE == 0)
         o.E_atom[g] = atom_energy(sp, yout);

@@ -139,6 +139,7 @@ struct aenet_amd_model {
   PotSet set;
   DeviceModel dev;
   DBuf<int> flags;  // error word of the training kernels
+  EnvScratch env;   // explicit environments (descriptor derivatives)
 };
 
 struct aenet_amd_mc {
@@ -1000,6 +1001,92 @@ void aenet_amd_model_weights(aenet_amd_model *model, float *d_W, int to_model, v
   hipError_t e = to_model ? hipMemcpyAsync(W, d_W, dm->wsize * sizeof(float), hipMemcpyDeviceToDevice, s)
                           : hipMemcpyAsync(d_W, W, dm->wsize * sizeof(float), hipMemcpyDeviceToDevice, s);
   *stat = e == hipSuccess ? AENET_STATUS_OK : set_err(AENET_STATUS_ERR_DEVICE, "weight copy failed");
+}
+
+void aenet_amd_model_descriptor_derivatives(aenet_amd_model *model, int nenv, const int env_start[],
+                                            const int type_i[], const double coo_i[], const double coo_j[],
+                                            const int type_j[], int stride, double values[], double deriv_i[],
+                                            double deriv_j[], int *stat) {
+  if (!model || nenv < 0 || !env_start || stride < model->dev.nsf_max) {
+    *stat = set_err(AENET_AMD_ERR_ARG_VALUE, "aenet_amd_model_descriptor_derivatives: bad arguments");
+    return;
+  }
+  const int nt = model->set.ntypes;
+  for (int e = 0; e < nenv; ++e) {
+    if (type_i[e] < 1 || type_i[e] > nt || env_start[e + 1] < env_start[e]) {
+      *stat = set_err(AENET_STATUS_ERR_TYPE, "aenet_amd_model_descriptor_derivatives: bad environment");
+      return;
+    }
+    for (int q = env_start[e]; q < env_start[e + 1]; ++q)
+      if (type_j[q] < 1 || type_j[q] > nt) {
+        *stat = set_err(AENET_STATUS_ERR_TYPE, "aenet_amd_model_descriptor_derivatives: type out of range");
+        return;
+      }
+  }
+  // environment e becomes nsf(e) copies, copy f differentiating feature f;
+  // chunks of environments keep the copies' neighbor data bounded
+  const hipStream_t s = lib_stream();
+  int e0 = 0;
+  while (e0 < nenv) {
+    int e1 = e0;
+    size_t copies = 0, nb = 0;
+    while (e1 < nenv) {
+      const int nsf = model->set.pots[type_i[e1] - 1].stp.nsf;
+      const size_t add = (size_t)nsf * (env_start[e1 + 1] - env_start[e1]);
+      if (e1 > e0 && (nb + add > (size_t)(4 << 20) || copies + nsf > 65536)) break;
+      copies += nsf;
+      nb += add;
+      ++e1;
+    }
+    // copies are laid out feature-major within an environment and padded
+    // to a multiple of coef_unit = stride, so copy c differentiates c % stride
+    const size_t nc = (size_t)(e1 - e0) * stride;
+    std::vector<int> start(nc + 1), ct(nc), tj;
+    std::vector<double> ci(3 * nc), cj;
+    size_t at = 0;
+    for (int e = e0; e < e1; ++e) {
+      const int n = env_start[e + 1] - env_start[e];
+      for (int f = 0; f < stride; ++f) {
+        const size_t c = (size_t)(e - e0) * stride + f;
+        start[c] = (int)at;
+        ct[c] = type_i[e];
+        std::memcpy(&ci[3 * c], &coo_i[3 * (size_t)e], 3 * sizeof(double));
+        // features beyond the species' nsf: an empty environment (nothing to differentiate)
+        if (f < model->set.pots[type_i[e] - 1].stp.nsf) {
+          cj.insert(cj.end(), coo_j + 3 * (size_t)env_start[e], coo_j + 3 * (size_t)env_start[e + 1]);
+          tj.insert(tj.end(), type_j + env_start[e], type_j + env_start[e + 1]);
+          at += n;
+        }
+      }
+    }
+    start[nc] = (int)at;
+    std::vector<double> E(nc), dFi(3 * nc), dFj(3 * at + 3), G(nc * stride);
+    std::string err;
+    int st = model->env.eval(model->dev, (int)nc, start.data(), ct.data(), ci.data(), cj.data(), tj.data(), true,
+                             E.data(), dFi.data(), dFj.data(), G.data(), stride, s, err, stride);
+    if (st) {
+      *stat = set_err(st, err);
+      return;
+    }
+    for (int e = e0; e < e1; ++e) {
+      const int n = env_start[e + 1] - env_start[e];
+      const size_t c0 = (size_t)(e - e0) * stride;
+      if (values) std::memcpy(&values[(size_t)e * stride], &G[c0 * stride], stride * sizeof(double));
+      for (int f = 0; f < stride; ++f) {
+        const size_t c = c0 + f;
+        const bool has = f < model->set.pots[type_i[e] - 1].stp.nsf;
+        for (int x = 0; x < 3; ++x) {
+          if (deriv_i) deriv_i[((size_t)e * stride + f) * 3 + x] = has ? -dFi[3 * c + x] : 0.0;
+          if (deriv_j)
+            for (int j = 0; j < n; ++j)
+              deriv_j[((size_t)(env_start[e] + j) * stride + f) * 3 + x] =
+                  has ? -dFj[3 * ((size_t)start[c] + j) + x] : 0.0;
+        }
+      }
+    }
+    e0 = e1;
+  }
+  *stat = AENET_STATUS_OK;
 }
 
 void aenet_amd_batch_create_model(const aenet_amd_model *model, int nstruct, const int natoms[], const double latt[],

@@ -75,6 +75,35 @@ class Model:
         _native.lib().aenet_amd_model_weights(self.handle, _ptr(W), 1, self._stream(), C.byref(stat))
         check_status(stat.value)
 
+    def descriptor_derivatives(self, envs, stride=None):
+        """Descriptor values and derivatives of explicit environments
+        (aenet_amd_model_descriptor_derivatives).  envs: list of (type_i,
+        coo_i (3,), coo_j (n, 3), type_j (n,)).  Returns (values [nenv,
+        stride], deriv_i [nenv, stride, 3], list of deriv_j [n, stride, 3])."""
+        stride = int(stride or self.nsf_max)
+        nenv = len(envs)
+        start = np.zeros(nenv + 1, np.int32)
+        for e, env in enumerate(envs):
+            start[e + 1] = start[e] + len(env[3])
+        ti = np.array([int(env[0]) for env in envs], np.int32)
+        ci = np.ascontiguousarray(np.concatenate([np.reshape(env[1], 3) for env in envs]), dtype=np.float64)
+        nb = int(start[-1])
+        cj = np.zeros(3 * max(nb, 1))
+        tj = np.zeros(max(nb, 1), np.int32)
+        for e, env in enumerate(envs):
+            if len(env[3]):
+                cj[3 * start[e]:3 * start[e + 1]] = np.ascontiguousarray(env[2], dtype=np.float64).reshape(-1)
+                tj[start[e]:start[e + 1]] = env[3]
+        vals = np.zeros((nenv, stride))
+        di = np.zeros((nenv, stride, 3))
+        dj = np.zeros((max(nb, 1), stride, 3))
+        stat = C.c_int()
+        _native.lib().aenet_amd_model_descriptor_derivatives(self.handle, nenv, start, ti, ci, cj, tj, stride,
+                                                            vals.ctypes.data, di.ctypes.data, dj.ctypes.data,
+                                                            C.byref(stat))
+        check_status(stat.value, "aenet_amd_model_descriptor_derivatives")
+        return vals, di, [dj[start[e]:start[e + 1]] for e in range(nenv)]
+
     def net_grad(self, G, types, E_ref=None, weight=None, upstream=None, coef=1.0, E_out=None, grad=None,
                  loss=None):
         """Atomic energies of normalized descriptors G (natoms, stride) and,

@@ -161,6 +161,21 @@ void aenet_amd_model_species_weights(const aenet_amd_model *model, int type_id,
 void aenet_amd_model_weights(aenet_amd_model *model, float *d_W, int to_model,
                              void *stream, int *stat);
 
+/* Descriptor values and their derivatives for explicit environments
+ * (what generate.x writes for force training, src/pytorchoutput.f90:105-139,
+ * from sfderiv_i / sfderiv_j of stp_eval): environment e is the central
+ * atom of species type_i[e] at coo_i[3e] with neighbors env_start[e] ..
+ * env_start[e+1]-1 of coo_j / type_j.  Out (host, may be NULL): values
+ * [nenv][stride] (unscaled), deriv_i [nenv][stride][3] (d/dR of the central
+ * atom), deriv_j [neighbor][stride][3] in the neighbors' order.  stride >=
+ * the model's largest feature count; features beyond a species' count are
+ * zero.  FP32 derivative sums (relative error ~1e-6).  Synchronous. */
+void aenet_amd_model_descriptor_derivatives(aenet_amd_model *model, int nenv,
+                                            const int env_start[], const int type_i[],
+                                            const double coo_i[], const double coo_j[],
+                                            const int type_j[], int stride, double values[],
+                                            double deriv_i[], double deriv_j[], int *stat);
+
 /* a batch planned for the model's cutoff */
 void aenet_amd_batch_create_model(const aenet_amd_model *model, int nstruct,
                                   const int natoms[], const double latt[],

@@ -223,3 +223,35 @@ def test_autograd_keeps_the_forward_weights(device):
     assert torch.equal(Ea, Er)
     assert torch.allclose(gxa, gxr, rtol=0, atol=1e-9)
     assert torch.allclose(gWa, gWr, rtol=1e-6, atol=1e-6)
+
+
+@pytest.mark.parametrize("name,types", [("alloy", ("A", "B")), ("mixed", ("A", "B")), ("bp3", ("A", "B", "C"))])
+def test_descriptor_derivatives_match_oracle(name, types, device):
+    """aenet_amd_model_descriptor_derivatives (the dG/dR export of
+    generate.x, src/pytorchoutput.f90:105-139) against the oracle's
+    unscaled descriptor derivatives (pinned to the reference through
+    bit-identical forces and central differences), Chebyshev and
+    Behler-Parrinello G1-G5, several environments in one call."""
+    from aenet_amd.model import Model
+    from oracle_lib import Oracle
+    files = pot_files(name, list(types))
+    m = Model(files, device=device)
+    o = Oracle(list(types), files)
+    rng = np.random.Generator(np.random.PCG64(12))
+    envs = []
+    for e in range(5):
+        coo = rng.uniform(-4.2, 4.2, size=(30 + 3 * e, 3))
+        coo = coo[np.linalg.norm(coo, axis=1) > 1.4]
+        envs.append((1 + e % len(types), rng.normal(0, 0.1, 3), coo, rng.integers(1, len(types) + 1, len(coo))))
+    vals, di, dj = m.descriptor_derivatives(envs)
+    worst = 0.0
+    for e, (t, c0, coo, tj) in enumerate(envs):
+        G, dGi, dGj = o.descriptors(t, c0, coo, tj, scaled=False, derivs=True)
+        nsf = len(G)
+        assert np.allclose(vals[e, :nsf], G, rtol=2e-6, atol=2e-5)
+        assert not vals[e, nsf:].any()
+        scale = max(1.0, np.abs(dGj).max())
+        err = max(np.abs(di[e, :nsf] - dGi).max(), np.abs(dj[e][:, :nsf] - dGj).max()) / scale
+        worst = max(worst, err)
+    print("%s: max |d dG/dR| / max(1, |dG/dR|) = %.3e" % (name, worst))
+    assert worst < 2e-5
