@@ -81,6 +81,42 @@ class Atoms:
         return self.calc.get_forces(self)
 
 
+def periodic_cell_for(cell, pbc, positions, cutoff, margin=1.0):
+    """A fully periodic cell equivalent to a partly periodic structure: the
+    periodic lattice vectors are kept, and each non-periodic one becomes a
+    vector perpendicular to the periodic ones, longer than the structure's
+    extent along it plus the cutoff, so that no periodic image along it
+    comes within the cutoff of any atom (every image displacement then has
+    a component of at least that length perpendicular to the periodic
+    vectors).  The atoms' environments, and so all energies and forces, are
+    those of the partly periodic structure."""
+    cell = np.asarray(cell, dtype=np.float64).reshape(3, 3)
+    pbc = np.asarray(pbc, dtype=bool)
+    per = [cell[k] for k in range(3) if pbc[k]]
+    # an orthonormal basis of the directions perpendicular to the periodic vectors
+    if len(per) == 2:
+        n = np.cross(per[0], per[1])
+        perp = [n / np.linalg.norm(n)]
+    elif len(per) == 1:
+        a = per[0] / np.linalg.norm(per[0])
+        t = np.eye(3)[int(np.argmin(np.abs(a)))]
+        e1 = np.cross(a, t)
+        e1 /= np.linalg.norm(e1)
+        perp = [e1, np.cross(a, e1)]
+    else:
+        raise ValueError("periodic_cell_for needs one or two periodic directions")
+    out = cell.copy()
+    x = np.asarray(positions, dtype=np.float64).reshape(-1, 3)
+    j = 0
+    for k in range(3):
+        if not pbc[k]:
+            e = perp[j]
+            j += 1
+            proj = x @ e
+            out[k] = e * ((proj.max() - proj.min()) + cutoff + margin)
+    return out
+
+
 class ANNCalculator(_Base):
 
     implemented_properties = ["energy", "forces"]
@@ -107,12 +143,15 @@ class ANNCalculator(_Base):
         _Base.calculate(self, atoms, properties, system_changes)
         atoms = self.atoms if atoms is None else atoms
         pbc = np.asarray(atoms.get_pbc(), dtype=bool)
+        cell = np.asarray(atoms.get_cell(), dtype=np.float64).reshape(3, 3)
         if pbc.any() and not pbc.all():
-            raise NotImplementedError("mixed periodic boundary conditions are not supported; "
-                                      "use a fully periodic cell with vacuum (as predict.x does)")
+            # partly periodic (slab, wire): the equivalent fully periodic
+            # cell, as the reference's ASE neighbor list treats them
+            # (python3/aenet/ase_calculator.py:49-56)
+            cell = periodic_cell_for(cell, pbc, atoms.get_positions(), self.cutoff + 1.0e-3)
+            pbc = np.ones(3, bool)
         want_forces = "forces" in properties
-        r = self.ann.structure_energy_and_forces(np.asarray(atoms.get_cell(), dtype=np.float64).reshape(3, 3),
-                                                 bool(pbc.all()), np.asarray(atoms.get_positions()),
+        r = self.ann.structure_energy_and_forces(cell, bool(pbc.all()), np.asarray(atoms.get_positions()),
                                                  atoms.get_chemical_symbols(), forces=want_forces)
         self.results["energy"] = float(r["E_tot"][0])
         self.results["energies"] = r["energies"]

@@ -389,3 +389,58 @@ def test_sfb_eval_deriv_matches_oracle(device):
         assert np.abs(dGj).max() > 0.1
     finally:
         L.aenet_sfb_final(C.byref(stat))
+
+
+@pytest.mark.parametrize("pbc", [(True, True, False), (True, False, False)], ids=["slab", "wire"])
+def test_ase_calculator_mixed_pbc(pbc, device):
+    """Partly periodic structures in the ASE-style calculator (the
+    reference's calculator builds their neighbor lists with ASE,
+    python3/aenet/ase_calculator.py:49-56) against the oracle's per-atom
+    energies and forces on brute-force neighbor lists that take images
+    along the periodic directions only."""
+    from aenet_amd.ase_calculator import ANNCalculator, Atoms
+    from aenet_amd import synth
+    from oracle_lib import Oracle
+    names = ["A", "B"]
+    files = pot_files("alloy", names)
+    s = synth.binary_fcc(3.8, (3, 3, 2), 0.1, 91, 92)
+    cell = s.cell.copy()
+    if pbc == (True, False, False):
+        cell[1] = 0.0
+        cell[2] = 0.0
+    else:
+        cell[2] = 0.0
+    atoms = Atoms([names[t - 1] for t in s.types], s.positions, cell, np.array(pbc))
+    calc = ANNCalculator(files)
+    try:
+        atoms.set_calculator(calc)
+        E = atoms.get_potential_energy()
+        F = atoms.get_forces()
+        Ei = calc.results["energies"]
+    finally:
+        calc.release()
+    o = Oracle(names, files)
+    rc2 = (o.Rc_max + 1e-3) ** 2
+    n = len(s.types)
+    reps = [range(-3, 4) if p else range(0, 1) for p in pbc]
+    Fref = np.zeros((n, 3))
+    Eref = np.zeros(n)
+    for i in range(n):
+        cj, tj, ij = [], [], []
+        for j in range(n):
+            for a in reps[0]:
+                for b in reps[1]:
+                    for c in reps[2]:
+                        if i == j and a == b == c == 0:
+                            continue
+                        x = s.positions[j] + a * s.cell[0] + b * s.cell[1] + c * s.cell[2]
+                        if np.sum((x - s.positions[i]) ** 2) <= rc2:
+                            cj.append(x)
+                            tj.append(s.types[j])
+                            ij.append(j + 1)
+        Eref[i] = o.atomic_energy_and_forces(s.positions[i], int(s.types[i]), i + 1, np.array(cj), tj, ij, Fref)
+    dE = np.abs(Ei - Eref).max()
+    dF = np.abs(F - Fref).max()
+    print("%s: max |dE| %.3e eV/atom  max |dF| %.3e eV/A" % (pbc, dE, dF))
+    assert dE < E_TOL and dF < F_TOL
+    assert abs(E - Eref.sum()) < E_TOL * n
