@@ -75,6 +75,7 @@ SIGNATURES = {
     "aenet_amd_batch_create_model": (None, [_vp, C.c_int, _ip, _dp, _ip, _dp, C.POINTER(_vp), _pint]),
     "aenet_amd_batch_eval_model": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
     "aenet_amd_batch_autotune": (None, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _pint, _pint, _vp, _pint]),
+    "aenet_amd_batch_set_launch": (None, [_vp, _vp, C.c_int, C.c_int, C.c_int, _pint]),
     "aenet_amd_batch_eval_active": (None, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _pint]),
     "aenet_amd_batch_vjp": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
     "aenet_amd_batch_descriptors_model": (None, [_vp, _vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _pint]),

@@ -100,6 +100,10 @@ struct AtomSizes {
   int tune_waves;    // wavefronts per workgroup
   int tune_staged;   // network weights staged in LDS (0 / 1)
   int nnorm;         // normalization entries of all species (staged with the weights)
+  // every species has the same layer count and widths, one output node:
+  // the networks of a workgroup's atoms can run as one matrix tile
+  int net_uniform;
+  int tune_tile;     // that tile network chosen by measurement (-1: not measured, off)
 };
 
 // network training (train.hip): per-atom normalized descriptors in, atomic

@@ -11,9 +11,14 @@
 //                   (first level FP32, then FP64) that leaves feature f in
 //                   lane f
 //   3. normalize  : stp_normalize (src/sfsetup.f90:985-1035) in FP64
-//   4. network    : ff_eval / ff_deriv (src/ext/feedforward.f90:586-757),
-//                   lanes = nodes of a layer, weights of every species held
-//                   in LDS for the whole workgroup
+//   4. network    : ff_eval / ff_deriv (src/ext/feedforward.f90:586-757).
+//                   Structures with one network architecture for every
+//                   species: the atoms of a workgroup round form the rows
+//                   of 16-row matrix tiles (v_mfma_f32_16x16x4f32), each
+//                   weight read once per round instead of once per atom.
+//                   Otherwise per wavefront, lanes = nodes of a layer.
+//                   Weights of every species held in LDS for the whole
+//                   workgroup (or read through the caches)
 //   5. forces     : F_j -= E_scale dy/dG dG/dR_j (src/aenet.f90:471-477):
 //                   pair and triplet derivatives contracted with dE/dG in
 //                   FP32, per-neighbor sums in LDS, FP64 atomics into F
@@ -37,6 +42,7 @@ constexpr int AUXN = 64;  // angular neighbors whose per-neighbor terms are tabu
 // memory; cap = neighbor capacity.
 struct WaveLds {
   SpeciesDev *spc;  // the central atom's species record
+  int *row;    // [1] the atom's species in the round's network tile (0: none)
   float4 *aux; // [AUXN] per angular neighbor: 1/r, fc, dfc/dr, typespin
   float *R;    // [3*cap]
   float *r;    // [cap]
@@ -60,6 +66,7 @@ struct ChebLaunch {
   const int *nactive;
   int tune_waves, tune_staged;  // host side: measured launch shape
   int nnorm;     // normalization entries (averages and scales staged after the weights)
+  int tile_net;  // networks of a round as one matrix tile (mode 0, one architecture)
 };
 
 // words staged once per workgroup: the weights (rounded to 4 words), then
@@ -67,7 +74,7 @@ struct ChebLaunch {
 __host__ __device__ inline int staged_words(const ChebLaunch &P) { return (P.wsize + 3) / 4 * 4 + 4 * P.nnorm; }
 
 __host__ __device__ inline int wave_lds_words(const ChebLaunch &P) {
-  return (SPECIES_WORDS + 4 * AUXN + 10 * P.cap + net_lds_words(P.nsf, P.nl, P.hs) + 3) / 4 * 4;
+  return (SPECIES_WORDS + 4 + 4 * AUXN + 10 * P.cap + net_lds_words(P.nsf, P.nl, P.hs) + 3) / 4 * 4;
 }
 
 __device__ inline WaveLds carve(float *base, const ChebLaunch &P) {
@@ -75,6 +82,8 @@ __device__ inline WaveLds carve(float *base, const ChebLaunch &P) {
   WaveLds w;
   w.spc = reinterpret_cast<SpeciesDev *>(base);
   base += SPECIES_WORDS;
+  w.row = reinterpret_cast<int *>(base);
+  base += 4;
   w.aux = reinterpret_cast<float4 *>(base);
   w.R = base + 4 * AUXN;
   w.r = w.R + 3 * cap;
@@ -86,14 +95,166 @@ __device__ inline WaveLds carve(float *base, const ChebLaunch &P) {
   return w;
 }
 
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// The networks of one round of a workgroup (ff_eval / ff_deriv,
+// src/ext/feedforward.f90:586-757, with the chain through the
+// normalization and the energy scale as network() has it) on the matrix
+// cores.  Row q of a 16-row tile is wavefront q's atom; its input (the
+// normalized descriptor), activations and back-propagation buffers stay in
+// wavefront q's LDS region, and *row holds its species (0: no atom).  A
+// task is one 16-column tile of one layer for one species present in the
+// round: rows of other species enter as zeros and their results are
+// dropped.  Every species has the same layer widths (nodes[], layer count
+// nl, one output node); offsets, activations and energy scale come from
+// the species record of the species' first row.
+//   forward : Z[q][j] = sum_k X[q][k] W_l[k][j] + b_l[j], K = n_l
+//   backward: D_l[q][k] = sum_j D_l+1[q][j] W_l[k][j] (times f'_l, or, for
+//             the input layer, E_scale times the normalization scale)
+// v_mfma_f32_16x16x4f32 operands: lane L gives A[L % 16][L / 16] and
+// B[L / 16][L % 16] of each K step of 4; accumulator r of lane L is
+// D[4 (L / 16) + r][L % 16].
+// One workgroup barrier ahead of the tile (the rows' inputs and species),
+// one after each layer; every wavefront of the workgroup executes all of
+// them, with or without an atom.  On return each row's N.act holds its
+// activations and, with GRAD, N.coef holds dE/dG.
+template <bool GRAD, class WPtr>
+__device__ inline void network_rows(WPtr W, float *regions, const ChebLaunch &P, const int lane) {
+  typedef __attribute__((address_space(3))) float lfloat;
+  typedef const __attribute__((address_space(3))) SpeciesDev lspecies;
+  lfloat *R = (lfloat *)regions;
+  const int wave = threadIdx.x / WAVE;
+  const int nw = P.waves;
+  const int wl = wave_lds_words(P);
+  const int hs = P.hs;
+  const int nl = P.nl;
+  // word offsets within a region (the layout of carve / carve_net)
+  const int o_row = SPECIES_WORDS;
+  const int o_gp = SPECIES_WORDS + 4 + 4 * AUXN + 10 * P.cap;
+  const int o_coef = o_gp + P.nsf;
+  const int o_d0 = o_coef + P.nsf;
+  const int o_d1 = o_d0 + net_dmax(P.nsf, hs);
+  const int o_act = o_d1 + net_dmax(P.nsf, hs);
+  const int o_dact = o_act + nl * hs;
+  const int rq = lane & 15;  // A row and B/D column of this lane
+  const int kq = lane >> 4;  // A/B k of this lane within a step, D row group
+  __syncthreads();
+  const int myt = rq < nw ? reinterpret_cast<__attribute__((address_space(3))) int *>(R)[rq * wl + o_row] : 0;
+  // rows that are the first of their species
+  bool dup = false;
+#pragma unroll
+  for (int m = 0; m < 15; ++m) {
+    const int tm = __shfl(myt, m, WAVE);
+    if (m < rq && tm == myt) dup = true;
+  }
+  const unsigned long long lead = wave_ballot(lane < 16 && myt > 0 && !dup);
+  const int ns = popc64(lead);
+  // species of this lane's four D rows, gathered here with every lane
+  // active (a cross-lane read from a lane that a branch has switched off
+  // does not return that lane's value)
+  int ot[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) ot[r] = __shfl(myt, 4 * kq + r, WAVE);
+  auto nth_row = [&](int s) {
+    unsigned long long m = lead;
+    for (int c = 0; c < s; ++c) m &= m - 1;
+    return (int)__builtin_ctzll(m);
+  };
+  auto species = [&](int q) -> lspecies & { return *(lspecies *)(R + q * wl); };
+  const int arow_base = rq * wl;        // this lane's A row (read only when of the task's species)
+  const int drow_base = 4 * kq * wl;    // first of this lane's four D rows
+
+  for (int l = 0; l + 1 < nl; ++l) {
+    if (ns > 0) {
+      lspecies &sp0 = species(nth_row(0));
+      const int n1 = sp0.nnodes[l], n2 = sp0.nnodes[l + 1];
+      const int ntile = (n2 + 15) >> 4;
+      const lfloat *X = R + arow_base + (l == 0 ? o_gp : o_act + l * hs);
+      for (int task = wave; task < ns * ntile; task += nw) {
+        const int s = task / ntile, jt = task - s * ntile;
+        const int lr = nth_row(s);
+        const int t = __shfl(myt, lr, WAVE);
+        lspecies &st = species(lr);
+        const WPtr Wl = W + st.w_off[l];
+        const int col = jt * 16 + rq;
+        const bool arow = myt == t;
+        floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k0 = 0; k0 < n1; k0 += 4) {
+          const int kk = k0 + kq;
+          const float a = (arow && kk < n1) ? X[kk] : 0.0f;
+          const float bw = (col < n2 && kk < n1) ? Wl[kk * n2 + col] : 0.0f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bw, acc, 0, 0, 0);
+        }
+        if (col < n2) {
+          const float bias = Wl[n1 * n2 + col];
+          const int fa = st.fa[l + 1];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (ot[r] == t) {
+              lfloat *D = R + drow_base + r * wl + (l + 1) * hs + col;
+              float y, dy;
+              activate(fa, acc[r] + bias, y, dy);
+              D[o_act] = y;
+              D[o_dact] = dy;
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (!GRAD) return;
+  for (int l = nl - 2; l >= 0; --l) {
+    if (ns > 0) {
+      lspecies &sp0 = species(nth_row(0));
+      const int n1 = sp0.nnodes[l], n2 = sp0.nnodes[l + 1];
+      const int ntile = (n1 + 15) >> 4;
+      // D of layer l+1: the output layer's f', else the buffer of layer
+      // l+1's parity; layer l's goes to the other one (the dE/dG words for
+      // l = 0)
+      const lfloat *src = R + arow_base + (l == nl - 2 ? o_dact + (nl - 1) * hs : ((l + 1) & 1 ? o_d1 : o_d0));
+      const int o_dst = l == 0 ? o_coef : (l & 1 ? o_d1 : o_d0);
+      for (int task = wave; task < ns * ntile; task += nw) {
+        const int s = task / ntile, jt = task - s * ntile;
+        const int lr = nth_row(s);
+        const int t = __shfl(myt, lr, WAVE);
+        lspecies &st = species(lr);
+        const WPtr Wl = W + st.w_off[l];
+        const int col = jt * 16 + rq;
+        const bool arow = myt == t;
+        floatx4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (int j0 = 0; j0 < n2; j0 += 4) {
+          const int jj = j0 + kq;
+          const float a = (arow && jj < n2) ? src[jj] : 0.0f;
+          const float bw = (col < n1 && jj < n2) ? Wl[col * n2 + jj] : 0.0f;
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bw, acc, 0, 0, 0);
+        }
+        if (col < n1) {
+          // for l = 0 the destination holds the normalization scale
+          const float m0 = l == 0 ? (float)st.E_scale : 1.0f;
+          const int o_mul = l == 0 ? o_coef : o_dact + l * hs;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (ot[r] == t) {
+              lfloat *D = R + drow_base + r * wl + col;
+              D[o_dst] = m0 * acc[r] * D[o_mul];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // MODE 0: structures (compact lists, FP64 atomic force scatter)
 // MODE 1: explicit environments (per-neighbor forces out)
-template <int RN, int AN, int MODE, bool FORCES>
+// TILE: the network of a round's atoms as one matrix tile (MODE 0)
+template <int RN, int AN, int MODE, bool FORCES, bool TILE>
 __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev md, BatchDev b, NbDev nbl, OutDev o, EnvDev env,
                                                           ChebLaunch P) {
   static_assert(2 * (RN + AN) <= 64, "features of one chunk");
   extern __shared__ float lds[];
-  const int lane = lane_id();
   const int wave = threadIdx.x / WAVE;
   const int nw = P.waves;
   // weights and normalization of every species, once per workgroup
@@ -111,146 +272,259 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
   WaveLds S = carve(lds + wsz + wave * wave_lds_words(P), P);
 
   const int count = P.active ? *P.nactive : P.natoms;
-  for (int k = blockIdx.x * nw + wave; k < count; k += gridDim.x * nw) {
-    const int g = P.active ? P.active[k] : k;
+  // rounds of gridDim.x * nw entries: wavefront w of workgroup b takes
+  // entry b * nw + w of each round.  With the tile network every wavefront
+  // runs every round of its workgroup (the tile's barriers), with or
+  // without an atom.
+  constexpr bool tile = MODE == 0 && TILE;
+  const int first = blockIdx.x * nw, per_round = gridDim.x * nw;
+  const int nrounds = count > first ? (count - first + per_round - 1) / per_round : 0;
+  for (int rd = 0; rd < nrounds; ++rd) {
+    // the lane index made opaque once per round: lane masks are computed
+    // where they are used instead of being hoisted out of the loop and
+    // held in scalar registers across it
+    int lane = lane_id();
+    asm volatile("" : "+v"(lane));
+    const int k = first + rd * per_round + wave;
+    const bool live = k < count;
+    if (!tile && !live) break;
+    const int g = live ? (P.active ? P.active[k] : k) : 0;
     wave_sync();
-    // ------------------------------------------------------------ 1
-    int ti, n, spw;
-    if (MODE == 0) {
-      // Two dependent memory rounds per atom: (type, count, list slots),
-      // which depend on g alone, then (species record, neighbor types).
-      // Every slot up to the capacity is read (all in bounds; those beyond
-      // the count go unused), so the list loads do not wait for the count.
-      ti = b.types[g];
-      const int cnt = nbl.count[g];
-      const float4 *R4 = reinterpret_cast<const float4 *>(nbl.R) + (size_t)g * nbl.cap;
-      const int *J = nbl.j + (size_t)g * nbl.cap;
-      auto load_slots = [&](int q0, float4 (&v)[2], int (&j)[2]) {
+    // phases 1-3; false for an atom of an unknown species
+    int ti = 0, n = 0, na = 0;
+    const bool front_ok = live && [&]() -> bool {
+      // ------------------------------------------------------------ 1
+      int spw;
+      if (MODE == 0) {
+        // Two dependent memory rounds per atom: (type, count, list slots),
+        // which depend on g alone, then (species record, neighbor types).
+        // Every slot up to the capacity is read (all in bounds; those beyond
+        // the count go unused), so the list loads do not wait for the count.
+        ti = b.types[g];
+        const int cnt = nbl.count[g];
+        const float4 *R4 = reinterpret_cast<const float4 *>(nbl.R) + (size_t)g * nbl.cap;
+        const int *J = nbl.j + (size_t)g * nbl.cap;
+        auto load_slots = [&](int q0, float4 (&v)[2], int (&j)[2]) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int q = q0 + u * WAVE + lane;
-          v[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-          j[u] = 0;
-          if (q < P.cap) {
-            v[u] = R4[q];
-            j[u] = J[q];
+          for (int u = 0; u < 2; ++u) {
+            const int q = q0 + u * WAVE + lane;
+            v[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            j[u] = 0;
+            if (q < P.cap) {
+              v[u] = R4[q];
+              j[u] = J[q];
+            }
           }
-        }
-      };
-      auto gather_types = [&](int q0, const int (&j)[2], int (&tj)[2]) {
+        };
+        auto gather_types = [&](int q0, const int (&j)[2], int (&tj)[2]) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          // unconditional (slots past the count read the atom's own type),
-          // so both gathers issue back to back
-          const int jj = q0 + u * WAVE + lane < n ? j[u] : g;
-          tj[u] = b.types[jj];
-        }
-      };
-      auto put_slots = [&](int q0, const float4 (&v)[2], const int (&j)[2], const int (&tj)[2]) {
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          const int q = q0 + u * WAVE + lane;
-          if (q < n) {
-            S.R[3 * q] = v[u].x;
-            S.R[3 * q + 1] = v[u].y;
-            S.R[3 * q + 2] = v[u].z;
-            S.r[q] = v[u].w;
-            S.j[q] = j[u];
-            S.t[q] = tj[u];
+          for (int u = 0; u < 2; ++u) {
+            // unconditional (slots past the count read the atom's own type),
+            // so both gathers issue back to back
+            const int jj = q0 + u * WAVE + lane < n ? j[u] : g;
+            tj[u] = b.types[jj];
           }
+        };
+        auto put_slots = [&](int q0, const float4 (&v)[2], const int (&j)[2], const int (&tj)[2]) {
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int q = q0 + u * WAVE + lane;
+            if (q < n) {
+              S.R[3 * q] = v[u].x;
+              S.R[3 * q + 1] = v[u].y;
+              S.R[3 * q + 2] = v[u].z;
+              S.r[q] = v[u].w;
+              S.j[q] = j[u];
+              S.t[q] = tj[u];
+            }
+          }
+        };
+        float4 v[2];
+        int j[2], tj[2];
+        load_slots(0, v, j);
+        n = min(cnt, P.cap);
+        gather_types(0, j, tj);
+        // issued after the gathers: loads complete in order, so the stores
+        // below wait for the gathers without waiting for this one
+        spw = species_word(md, min(max(ti, 1), md.ntypes) - 1);  // clamped: a bad type is refused below
+        put_slots(0, v, j, tj);
+        for (int q0 = 2 * WAVE; q0 < P.cap; q0 += 2 * WAVE) {
+          load_slots(q0, v, j);
+          gather_types(q0, j, tj);
+          put_slots(q0, v, j, tj);
         }
-      };
-      float4 v[2];
-      int j[2], tj[2];
-      load_slots(0, v, j);
-      n = min(cnt, P.cap);
-      gather_types(0, j, tj);
-      // issued after the gathers: loads complete in order, so the stores
-      // below wait for the gathers without waiting for this one
-      spw = species_word(md, min(max(ti, 1), md.ntypes) - 1);  // clamped: a bad type is refused below
-      put_slots(0, v, j, tj);
-      for (int q0 = 2 * WAVE; q0 < P.cap; q0 += 2 * WAVE) {
-        load_slots(q0, v, j);
-        gather_types(q0, j, tj);
-        put_slots(q0, v, j, tj);
+      } else {
+        ti = env.env_type[g];
+        spw = species_word(md, min(max(ti, 1), md.ntypes) - 1);
+        int e0 = env.env_start[g];
+        n = env.env_start[g + 1] - e0;
+        if (n > P.cap) {
+          if (lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_NB_OVERFLOW);
+          n = P.cap;
+        }
+        const double ci[3] = {env.coo_i[3 * g], env.coo_i[3 * g + 1], env.coo_i[3 * g + 2]};
+        for (int q = lane; q < n; q += WAVE) {
+          double c[3];
+          for (int k = 0; k < 3; ++k) c[k] = env.coo_j[3 * (size_t)(e0 + q) + k] - ci[k];
+          S.R[3 * q] = (float)c[0];
+          S.R[3 * q + 1] = (float)c[1];
+          S.R[3 * q + 2] = (float)c[2];
+          S.r[q] = (float)sqrt(dist2_exact(c));
+          S.j[q] = e0 + q;
+          S.t[q] = env.type_j[e0 + q];
+        }
       }
-    } else {
-      ti = env.env_type[g];
-      spw = species_word(md, min(max(ti, 1), md.ntypes) - 1);
-      int e0 = env.env_start[g];
-      n = env.env_start[g + 1] - e0;
-      if (n > P.cap) {
-        if (lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_NB_OVERFLOW);
-        n = P.cap;
+      ti = uniform(ti);
+      n = uniform(n);
+      if (ti < 1 || ti > md.ntypes) {
+        if (lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_BAD_TYPE);
+        return false;
       }
-      const double ci[3] = {env.coo_i[3 * g], env.coo_i[3 * g + 1], env.coo_i[3 * g + 2]};
-      for (int q = lane; q < n; q += WAVE) {
-        double c[3];
-        for (int k = 0; k < 3; ++k) c[k] = env.coo_j[3 * (size_t)(e0 + q) + k] - ci[k];
-        S.R[3 * q] = (float)c[0];
-        S.R[3 * q + 1] = (float)c[1];
-        S.R[3 * q + 2] = (float)c[2];
-        S.r[q] = (float)sqrt(dist2_exact(c));
-        S.j[q] = e0 + q;
-        S.t[q] = env.type_j[e0 + q];
-      }
-    }
-    ti = uniform(ti);
-    n = uniform(n);
-    if (ti < 1 || ti > md.ntypes) {
-      if (lane == 0) atomicOr(b.flags, AENET_AMD_FLAG_BAD_TYPE);
-      continue;
-    }
-    wave_sync();
-    const SpeciesDev &sp = store_species(spw, S.spc);
-    const int rN = sp.r_N, aN = sp.a_N, nsf = sp.nsf;
-    const bool multi = sp.multi != 0;
-    const float rRc = sp.r_Rc, aRc = sp.a_Rc;
-    const float inv_rRc = 1.0f / rRc;
-    const float inv_aRc = aRc > 0.0f ? 1.0f / aRc : 0.0f;
+      wave_sync();
+      const SpeciesDev &sp = store_species(spw, S.spc);
+      const int rN = sp.r_N, aN = sp.a_N;
+      const bool multi = sp.multi != 0;
+      const float rRc = sp.r_Rc, aRc = sp.a_Rc;
+      const float inv_rRc = 1.0f / rRc;
+      const float inv_aRc = aRc > 0.0f ? 1.0f / aRc : 0.0f;
 
-    // ------------------------------------------------------------ 2
-    // accumulators in template layout: [radial RN | angular AN |
-    // radial*spin RN | angular*spin AN | padding]
-    float acc[64];
+      // ------------------------------------------------------------ 2
+      // accumulators in template layout: [radial RN | angular AN |
+      // radial*spin RN | angular*spin AN | padding]
+      float acc[64];
 #pragma unroll
-    for (int k = 0; k < 64; ++k) acc[k] = 0.0f;
-    for (int q = lane; q < n; q += WAVE) {
-      float r = S.r[q];
-      if (r <= rRc && r > EPS_R) {
+      for (int k = 0; k < 64; ++k) acc[k] = 0.0f;
+      for (int q = lane; q < n; q += WAVE) {
+        float r = S.r[q];
+        if (r <= rRc && r > EPS_R) {
+          float fc, dfc;
+          cutoff(r, rRc, inv_rRc, fc, dfc);
+          float x = 2.0f * r * inv_rRc - 1.0f;
+          float sfc = sp.spin[S.t[q] - 1] * fc;
+          float Tm = 1.0f, T = x;
+          acc[0] += fc;
+          acc[RN + AN] += sfc;
+#pragma unroll
+          for (int k = 1; k < RN; ++k) {
+            acc[k] += fc * T;
+            acc[RN + AN + k] += sfc * T;
+            float Tn = 2.0f * x * T - Tm;
+            Tm = T;
+            T = Tn;
+          }
+        }
+      }
+      // angular neighbor list (order preserved)
+      na = 0;
+      for (int base = 0; base < n; base += WAVE) {
+        int q = base + lane;
+        bool in = false;
+        if (q < n) {
+          float r = S.r[q];
+          in = r <= aRc && r >= EPS_R;
+        }
+        unsigned long long m = wave_ballot(in);
+        if (in) S.ang[na + mask_prefix(m)] = q;
+        na += popc64(m);
+      }
+      wave_sync();
+      // per angular neighbor: 1/r, cutoff and its derivative, typespin --
+      // tabulated once per atom for both triplet passes
+      const bool use_aux = na <= AUXN;
+      auto ang_terms = [&](int q) -> float4 {
+        const float r = S.r[q];
         float fc, dfc;
-        cutoff(r, rRc, inv_rRc, fc, dfc);
-        float x = 2.0f * r * inv_rRc - 1.0f;
-        float sfc = sp.spin[S.t[q] - 1] * fc;
-        float Tm = 1.0f, T = x;
-        acc[0] += fc;
-        acc[RN + AN] += sfc;
+        cutoff(r, aRc, inv_aRc, fc, dfc);
+        return make_float4(1.0f / r, fc, dfc, multi ? sp.spin[S.t[q] - 1] : 0.0f);
+      };
+      if (use_aux) {
+        if (lane < na) S.aux[lane] = ang_terms(S.ang[lane]);
+        wave_sync();
+      }
+      auto ang_get = [&](int a, int q) -> float4 { return use_aux ? S.aux[a] : ang_terms(q); };
+      const int ntrip = na * (na - 1) / 2;
+      for (int p = lane; p < ntrip; p += WAVE) {
+        int a, c;
+        tri_decode_diag(p, na, a, c);
+        int qj = S.ang[a], qk = S.ang[c];
+        const float4 Aj = ang_get(a, qj), Ak = ang_get(c, qk);
+        float cosv = (S.R[3 * qj] * S.R[3 * qk] + S.R[3 * qj + 1] * S.R[3 * qk + 1] + S.R[3 * qj + 2] * S.R[3 * qk + 2]) *
+                     (Aj.x * Ak.x);
+        float w = Aj.y * Ak.y;
+        float sw = Aj.w * Ak.w * w;
+        float Tm = 1.0f, T = cosv;
+        acc[RN] += w;
+        acc[2 * RN + AN] += sw;
 #pragma unroll
-        for (int k = 1; k < RN; ++k) {
-          acc[k] += fc * T;
-          acc[RN + AN + k] += sfc * T;
-          float Tn = 2.0f * x * T - Tm;
+        for (int k = 1; k < AN; ++k) {
+          acc[RN + k] += w * T;
+          acc[2 * RN + AN + k] += sw * T;
+          float Tn = 2.0f * cosv * T - Tm;
           Tm = T;
           T = Tn;
         }
       }
-    }
-    // angular neighbor list (order preserved)
-    int na = 0;
-    for (int base = 0; base < n; base += WAVE) {
-      int q = base + lane;
-      bool in = false;
-      if (q < n) {
-        float r = S.r[q];
-        in = r <= aRc && r >= EPS_R;
+      double G = reduce_scatter<64>(acc);
+      // lane f holds template feature f; map it to the species' layout
+      {
+        int f = -1;
+        if (lane < RN) {
+          if (lane < rN) f = sp.r_i1 + lane;
+        } else if (lane < RN + AN) {
+          if (lane - RN < aN) f = sp.a_i1 + lane - RN;
+        } else if (lane < 2 * RN + AN) {
+          if (multi && lane - RN - AN < rN) f = sp.r_i2 + lane - RN - AN;
+        } else if (lane < 2 * (RN + AN)) {
+          if (multi && lane - 2 * RN - AN < aN) f = sp.a_i2 + lane - 2 * RN - AN;
+        }
+        if (f >= 0) {
+          // ------------------------------------------------------ 3
+          // (staged copies read through LDS-typed pointers: DS loads)
+          typedef const __attribute__((address_space(3))) double *lds_f64;
+          const double sc = P.w_in_lds ? ((lds_f64)lnorm)[P.nnorm + sp.norm_off + f] : md.norm_scale[sp.norm_off + f];
+          const double av = P.w_in_lds ? ((lds_f64)lnorm)[sp.norm_off + f] : md.norm_avg[sp.norm_off + f];
+          double Gs = sc * (G - av);
+          S.net.gp[f] = (float)Gs;
+          S.net.coef[f] = (float)sc;  // for the network's dE/dG
+          if (o.G) o.G[(size_t)g * o.G_stride + f] = o.G_scaled ? Gs : G;
+        }
       }
-      unsigned long long m = wave_ballot(in);
-      if (in) S.ang[na + mask_prefix(m)] = q;
-      na += popc64(m);
+      wave_sync();
+      return true;
+    }();
+    const bool ok = uniform((int)front_ok) != 0;
+
+    // ------------------------------------------------------------ 4
+    float yout = 0.0f;
+    if constexpr (tile) {
+      {
+        if (lane == 0) *S.row = ok ? ti : 0;
+        if (P.w_in_lds)
+          network_rows<FORCES>((lds_weights)lds, lds + wsz, P, lane);
+        else
+          network_rows<FORCES>((global_weights)md.W, lds + wsz, P, lane);
+        if (!ok) continue;
+        yout = S.net.act[(S.spc->nlayers - 1) * S.net.hs];
+      }
+    } else {
+      if (!ok || o.desc_only) continue;
+      const SpeciesDev &sp = *S.spc;
+      if (MODE == 1 && o.coef_unit > 0) {
+        // derivatives of one descriptor feature instead of the energy
+        const int fu = g % o.coef_unit;
+        for (int f = lane; f < sp.nsf; f += WAVE) S.net.coef[f] = f == fu ? 1.0f : 0.0f;
+        wave_sync();
+      } else {
+        yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES, true)
+                          : network(md, sp, (global_weights)md.W, S.net, FORCES, true);
+      }
     }
-    wave_sync();
-    // per angular neighbor: 1/r, cutoff and its derivative, typespin --
-    // tabulated once per atom for both triplet passes
+    const SpeciesDev &sp = *S.spc;
+    const int rN = sp.r_N, aN = sp.a_N;
+    const bool multi = sp.multi != 0;
+    const float rRc = sp.r_Rc, aRc = sp.a_Rc;
+    const float inv_rRc = 1.0f / rRc;
+    const float inv_aRc = aRc > 0.0f ? 1.0f / aRc : 0.0f;
     const bool use_aux = na <= AUXN;
     auto ang_terms = [&](int q) -> float4 {
       const float r = S.r[q];
@@ -258,73 +532,8 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
       cutoff(r, aRc, inv_aRc, fc, dfc);
       return make_float4(1.0f / r, fc, dfc, multi ? sp.spin[S.t[q] - 1] : 0.0f);
     };
-    if (use_aux) {
-      if (lane < na) S.aux[lane] = ang_terms(S.ang[lane]);
-      wave_sync();
-    }
     auto ang_get = [&](int a, int q) -> float4 { return use_aux ? S.aux[a] : ang_terms(q); };
     const int ntrip = na * (na - 1) / 2;
-    for (int p = lane; p < ntrip; p += WAVE) {
-      int a, c;
-      tri_decode_diag(p, na, a, c);
-      int qj = S.ang[a], qk = S.ang[c];
-      const float4 Aj = ang_get(a, qj), Ak = ang_get(c, qk);
-      float cosv = (S.R[3 * qj] * S.R[3 * qk] + S.R[3 * qj + 1] * S.R[3 * qk + 1] + S.R[3 * qj + 2] * S.R[3 * qk + 2]) *
-                   (Aj.x * Ak.x);
-      float w = Aj.y * Ak.y;
-      float sw = Aj.w * Ak.w * w;
-      float Tm = 1.0f, T = cosv;
-      acc[RN] += w;
-      acc[2 * RN + AN] += sw;
-#pragma unroll
-      for (int k = 1; k < AN; ++k) {
-        acc[RN + k] += w * T;
-        acc[2 * RN + AN + k] += sw * T;
-        float Tn = 2.0f * cosv * T - Tm;
-        Tm = T;
-        T = Tn;
-      }
-    }
-    double G = reduce_scatter<64>(acc);
-    // lane f holds template feature f; map it to the species' layout
-    {
-      int f = -1;
-      if (lane < RN) {
-        if (lane < rN) f = sp.r_i1 + lane;
-      } else if (lane < RN + AN) {
-        if (lane - RN < aN) f = sp.a_i1 + lane - RN;
-      } else if (lane < 2 * RN + AN) {
-        if (multi && lane - RN - AN < rN) f = sp.r_i2 + lane - RN - AN;
-      } else if (lane < 2 * (RN + AN)) {
-        if (multi && lane - 2 * RN - AN < aN) f = sp.a_i2 + lane - 2 * RN - AN;
-      }
-      if (f >= 0) {
-        // ------------------------------------------------------ 3
-        // (staged copies read through LDS-typed pointers: DS loads)
-        typedef const __attribute__((address_space(3))) double *lds_f64;
-        const double sc = P.w_in_lds ? ((lds_f64)lnorm)[P.nnorm + sp.norm_off + f] : md.norm_scale[sp.norm_off + f];
-        const double av = P.w_in_lds ? ((lds_f64)lnorm)[sp.norm_off + f] : md.norm_avg[sp.norm_off + f];
-        double Gs = sc * (G - av);
-        S.net.gp[f] = (float)Gs;
-        S.net.coef[f] = (float)sc;  // for the network's dE/dG
-        if (o.G) o.G[(size_t)g * o.G_stride + f] = o.G_scaled ? Gs : G;
-      }
-    }
-    wave_sync();
-    if (o.desc_only) continue;
-
-    // ------------------------------------------------------------ 4
-    float yout;
-    if (MODE == 1 && o.coef_unit > 0) {
-      // derivatives of one descriptor feature instead of the energy
-      const int fu = g % o.coef_unit;
-      for (int f = lane; f < sp.nsf; f += WAVE) S.net.coef[f] = f == fu ? 1.0f : 0.0f;
-      wave_sync();
-      yout = 0.0f;
-    } else {
-      yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES, true)
-                        : network(md, sp, (global_weights)md.W, S.net, FORCES, true);
-    }
     if (lane == 0) {
       if (MODE == 0)
         o.E_atom[g] = atom_energy(sp, yout);
@@ -495,8 +704,12 @@ static void launch_one(const ModelDev &md, const BatchDev &b, const NbDev &nb, c
   }
   P.w_in_lds = ps.w_in_lds;
   P.waves = ps.waves;
-  hipLaunchKernelGGL((k_atom_cheb<RN, AN, MODE, F>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream, md, b, nb, o,
-                     e, P);
+  if (MODE == 0 && P.tile_net && !o.desc_only)
+    hipLaunchKernelGGL((k_atom_cheb<RN, AN, MODE, F, MODE == 0>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream,
+                       md, b, nb, o, e, P);
+  else
+    hipLaunchKernelGGL((k_atom_cheb<RN, AN, MODE, F, false>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream, md,
+                       b, nb, o, e, P);
 }
 
 template <int RN, int AN>
@@ -530,6 +743,8 @@ static ChebLaunch cheb_launch(int mode, const AtomSizes &z) {
   P.tune_waves = z.tune_waves;
   P.tune_staged = z.tune_staged;
   P.nnorm = z.nnorm;
+  // the matrix-tile network: structures, one architecture, chosen by measurement
+  P.tile_net = mode == 0 && z.net_uniform && z.tune_tile == 1;
   return P;
 }
 

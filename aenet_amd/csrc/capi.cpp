@@ -1126,7 +1126,7 @@ void aenet_amd_batch_eval_model(aenet_amd_batch *batch, const aenet_amd_model *m
 
 void aenet_amd_batch_autotune(aenet_amd_batch *batch, const aenet_amd_model *model, const double *d_coords,
                               const int *d_types, int forces, int repeats, double results[], int *ncand,
-                              int chosen[2], void *stream, int *stat) {
+                              int chosen[3], void *stream, int *stat) {
   const DeviceModel *dm = pick_model(model);
   if (!batch || !dm) {
     *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or no model");
@@ -1141,7 +1141,7 @@ void aenet_amd_batch_autotune(aenet_amd_batch *batch, const aenet_amd_model *mod
     return;
   }
   hipStream_t s = (hipStream_t)stream;  // NULL: the default stream
-  std::vector<std::array<double, 3>> out;
+  std::vector<std::array<double, 4>> out;
   std::string err;
   *stat = batch->batch.autotune(*dm, d_coords, d_types, forces != 0, repeats, s, out, err);
   if (*stat) {
@@ -1150,20 +1150,34 @@ void aenet_amd_batch_autotune(aenet_amd_batch *batch, const aenet_amd_model *mod
   }
   if (ncand) *ncand = (int)out.size();
   double best = 1e300;
-  int bw = 0, bs = -1;
+  int bw = 0, bs = -1, bt = 0;
   for (size_t k = 0; k < out.size(); ++k) {
-    if (results && k < 16)
-      for (int c = 0; c < 3; ++c) results[3 * k + c] = out[k][c];
-    if (out[k][2] < best) {
-      best = out[k][2];
+    if (results && k < 32)
+      for (int c = 0; c < 4; ++c) results[4 * k + c] = out[k][c];
+    if (out[k][3] < best) {
+      best = out[k][3];
       bw = (int)out[k][0];
       bs = (int)out[k][1];
+      bt = (int)out[k][2];
     }
   }
   if (chosen) {
     chosen[0] = bw;
     chosen[1] = bs;
+    chosen[2] = bt;
   }
+}
+
+void aenet_amd_batch_set_launch(aenet_amd_batch *batch, const aenet_amd_model *model, int waves, int staged,
+                                int tile, int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!batch || !dm) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or no model");
+    return;
+  }
+  std::string err;
+  *stat = batch->batch.set_launch(*dm, waves, staged, tile, err);
+  if (*stat) set_err(*stat, "aenet_amd_batch_set_launch: " + err);
 }
 
 void aenet_amd_batch_eval_active(aenet_amd_batch *batch, const aenet_amd_model *model, const double *d_coords,

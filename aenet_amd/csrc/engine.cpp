@@ -29,6 +29,13 @@ int DeviceModel::upload(const PotSet &ps, hipStream_t s) {
     nl_max = std::max(nl_max, sd.nlayers);
     for (int l = 1; l < sd.nlayers; ++l) hmax = std::max(hmax, sd.nnodes[l]);
   }
+  net_uniform = !ps.species.empty();
+  for (const SpeciesDev &sd : ps.species) {
+    const SpeciesDev &s0 = ps.species[0];
+    bool same = sd.nlayers == s0.nlayers && sd.nlayers >= 2 && sd.nnodes[sd.nlayers - 1] == 1;
+    for (int l = 0; same && l < sd.nlayers; ++l) same = sd.nnodes[l] == s0.nnodes[l];
+    if (!same) net_uniform = 0;
+  }
   if (species.upload(ps.species.data(), ps.species.size(), s)) return AENET_STATUS_ERR_DEVICE;
   if (W.upload(ps.W.data(), ps.W.size(), s)) return AENET_STATUS_ERR_DEVICE;
   if (avg.upload(ps.norm_avg.data(), ps.norm_avg.size(), s)) return AENET_STATUS_ERR_DEVICE;
@@ -438,14 +445,15 @@ AtomSizes Batch::sizes_for(const DeviceModel &dm, int natoms, const int *active,
   AtomSizes z = dm.sizes(natoms, nb_cap_, active, nactive);
   const auto it = tune_.find(&dm);
   if (it != tune_.end()) {
-    z.tune_waves = it->second.first;
-    z.tune_staged = it->second.second;
+    z.tune_waves = it->second[0];
+    z.tune_staged = it->second[1];
+    z.tune_tile = it->second[2];
   }
   return z;
 }
 
 int Batch::autotune(const DeviceModel &dm, const double *d_coords, const int *d_types, bool forces, int repeats,
-                    hipStream_t stream, std::vector<std::array<double, 3>> &out, std::string &err) {
+                    hipStream_t stream, std::vector<std::array<double, 4>> &out, std::string &err) {
   out.clear();
   repeats = std::max(1, std::min(repeats, 100));
   const int nat = plan_.natoms;
@@ -460,17 +468,21 @@ int Batch::autotune(const DeviceModel &dm, const double *d_coords, const int *d_
   (void)kernel_time(t0, c0);  // drop events of earlier evaluations
   tune_.erase(&dm);
   const int max_waves = dm.desc == DESC_CHEBYSHEV ? CHEB_MAX_WAVES : ATOM_MAX_WAVES;
+  // the matrix-tile network is a candidate where the Chebyshev kernel has it
+  const int max_tile = dm.desc == DESC_CHEBYSHEV && dm.net_uniform ? 1 : 0;
   double best = 1e300;
-  std::pair<int, int> choice(0, -1);
+  std::array<int, 3> choice = {0, -1, -1};
   int st = AENET_STATUS_OK;
+  for (int tile = 0; tile <= max_tile && !st; ++tile)
   for (int waves = 1; waves <= max_waves && !st; waves *= 2) {
     for (int staged = 0; staged <= 1 && !st; ++staged) {
       AtomSizes z = dm.sizes(nat, nb_cap_);
       z.tune_waves = waves;
       z.tune_staged = staged;
+      z.tune_tile = tile;
       const bool ok = dm.desc == DESC_CHEBYSHEV ? atom_cheb_shape_ok(z) : atom_bp_shape_ok(z);
       if (!ok) continue;
-      tune_[&dm] = std::make_pair(waves, staged);
+      tune_[&dm] = {waves, staged, tile};
       timing_ = false;
       st = eval(dm, d_coords, d_types, scratch_E_.p, E_struct.p, forces ? F.p : nullptr, stream);  // warm-up
       timing_ = true;
@@ -481,10 +493,10 @@ int Batch::autotune(const DeviceModel &dm, const double *d_coords, const int *d_
       if (!st) st = kernel_time(ms, count);
       if (!st && count > 0) {
         const double avg = ms / count;
-        out.push_back({(double)waves, (double)staged, avg});
+        out.push_back({(double)waves, (double)staged, (double)tile, avg});
         if (avg < best) {
           best = avg;
-          choice = std::make_pair(waves, staged);
+          choice = {waves, staged, tile};
         }
       }
     }
@@ -511,6 +523,33 @@ int Batch::autotune(const DeviceModel &dm, const double *d_coords, const int *d_
     return AENET_STATUS_ERR_DEVICE;
   }
   tune_[&dm] = choice;
+  return AENET_STATUS_OK;
+}
+
+int Batch::set_launch(const DeviceModel &dm, int waves, int staged, int tile, std::string &err) {
+  if (waves == 0) {
+    tune_.erase(&dm);
+    return AENET_STATUS_OK;
+  }
+  const bool cheb = dm.desc == DESC_CHEBYSHEV;
+  const int max_waves = cheb ? CHEB_MAX_WAVES : ATOM_MAX_WAVES;
+  if (waves < 0 || waves > max_waves || (waves & (waves - 1)) || staged < 0 || staged > 1 || tile < 0 || tile > 1) {
+    err = "launch shape out of range";
+    return AENET_AMD_ERR_ARG_VALUE;
+  }
+  if (tile && !(cheb && dm.net_uniform)) {
+    err = "the tile network needs Chebyshev descriptors and one network architecture for every species";
+    return AENET_AMD_ERR_ARG_VALUE;
+  }
+  AtomSizes z = dm.sizes(plan_.natoms, nb_cap_);
+  z.tune_waves = waves;
+  z.tune_staged = staged;
+  z.tune_tile = tile;
+  if (!(cheb ? atom_cheb_shape_ok(z) : atom_bp_shape_ok(z))) {
+    err = "launch shape does not fit the device";
+    return AENET_AMD_ERR_ARG_VALUE;
+  }
+  tune_[&dm] = {waves, staged, tile};
   return AENET_STATUS_OK;
 }
 

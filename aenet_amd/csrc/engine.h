@@ -88,10 +88,10 @@ struct DeviceModel {
   DBuf<double> avg, scale;
   DBuf<BPDev> bp;
   DBuf<BPFunc> bpf;
-  int ntypes = 0, desc = 0, max_rN = 0, max_aN = 0, nsf_max = 0, wsize = 0, nl_max = 0, hmax = 1, nnorm = 0;
+  int ntypes = 0, desc = 0, max_rN = 0, max_aN = 0, nsf_max = 0, wsize = 0, nl_max = 0, hmax = 1, nnorm = 0, net_uniform = 0;
   double rc_max = 0.0;
   AtomSizes sizes(int natoms, int cap, const int *active = nullptr, const int *nactive = nullptr) const {
-    return AtomSizes{natoms, cap, nsf_max, wsize, nl_max, hmax, active, nactive, 0, -1, nnorm};
+    return AtomSizes{natoms, cap, nsf_max, wsize, nl_max, hmax, active, nactive, 0, -1, nnorm, net_uniform, -1};
   }
   int upload(const PotSet &ps, hipStream_t s);
   ModelDev view() const;
@@ -141,7 +141,10 @@ class Batch {
   // kept for the batch's later evaluations with dm; out receives
   // (waves, staged, average ms) per candidate
   int autotune(const DeviceModel &dm, const double *d_coords, const int *d_types, bool forces, int repeats,
-               hipStream_t stream, std::vector<std::array<double, 3>> &out, std::string &err);
+               hipStream_t stream, std::vector<std::array<double, 4>> &out, std::string &err);
+  // a launch shape given by the caller (as one the tuner keeps); waves = 0
+  // returns to the occupancy model
+  int set_launch(const DeviceModel &dm, int waves, int staged, int tile, std::string &err);
   // HIP events around the atom kernel of every evaluation while enabled
   void set_timing(bool on);
   // summed duration (ms) and count of the timed atom-kernel launches since
@@ -165,7 +168,7 @@ class Batch {
   // atom-kernel sizes of dm with the batch's measured launch shape
   AtomSizes sizes_for(const DeviceModel &dm, int natoms, const int *active = nullptr,
                       const int *nactive = nullptr) const;
-  std::map<const DeviceModel *, std::pair<int, int>> tune_;  // model -> (waves, staged)
+  std::map<const DeviceModel *, std::array<int, 3>> tune_;  // model -> (waves, staged, tile network)
   int allocate(std::string &err);
   PlanHost plan_;
   double rc_max_ = 0.0;

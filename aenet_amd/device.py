@@ -183,21 +183,31 @@ class DeviceBatch:
     def autotune(self, forces=True, repeats=5, model=None, coords=None, types=None):
         """Time every feasible launch shape of the atom kernel on this batch
         (aenet_amd_batch_autotune) and keep the fastest for the later
-        evaluations with the model.  Returns ((waves, staged), rows) with
-        rows = [(waves, staged, average ms), ...]."""
+        evaluations with the model.  Returns ((waves, staged, tile), rows)
+        with rows = [(waves, staged, tile, average ms), ...]; tile = 1 for
+        the networks of a workgroup's atoms as one matrix-core tile."""
         coords = self.coords if coords is None else coords
         types = self.types if types is None else types
         self._check_inputs(coords, types)
-        res = np.zeros(48)
+        res = np.zeros(4 * 32)
         n = C.c_int(0)
-        chosen = (C.c_int * 2)()
+        chosen = (C.c_int * 3)()
         stat = C.c_int(0)
         _native.lib().aenet_amd_batch_autotune(self._h, self._model(model), _ptr(coords), _ptr(types), int(forces),
                                                int(repeats), res.ctypes.data, C.byref(n), chosen, self._stream(),
                                                C.byref(stat))
         check_status(stat.value, "aenet_amd_batch_autotune")
-        rows = [(int(res[3 * k]), int(res[3 * k + 1]), float(res[3 * k + 2])) for k in range(min(n.value, 16))]
-        return (chosen[0], chosen[1]), rows
+        rows = [(int(res[4 * k]), int(res[4 * k + 1]), int(res[4 * k + 2]), float(res[4 * k + 3]))
+                for k in range(min(n.value, 32))]
+        return (chosen[0], chosen[1], chosen[2]), rows
+
+    def set_launch(self, waves, staged=1, tile=0, model=None):
+        """Use this launch shape of the atom kernel (aenet_amd_batch_set_launch);
+        waves = 0 returns to the occupancy model."""
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_set_launch(self._h, self._model(model), int(waves), int(staged), int(tile),
+                                                 C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_set_launch")
 
     def descriptors(self, stride, scaled=False, coords=None, types=None, model=None):
         coords = self.coords if coords is None else coords

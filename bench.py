@@ -145,6 +145,14 @@ BP_WSIZE = 133 * 25 + 26 * 25 + 26
 BP_FLOP_ATOM = 4 * BP_WSIZE + 3 * 132
 
 
+def launch_shape(chosen, rows):
+    """The tuner's choice and every candidate it timed (aenet_amd_batch_autotune);
+    tile = 1: the networks of a workgroup's atoms as one matrix-core tile."""
+    return {"chosen": {"waves_per_workgroup": chosen[0], "weights_in_lds": chosen[1], "tile_network": chosen[2]},
+            "candidates_us": [{"waves_per_workgroup": w, "weights_in_lds": s, "tile_network": t, "kernel_us": ms * 1e3}
+                              for (w, s, t, ms) in rows]}
+
+
 def bp_counts(db, rc=6.0, chunk=2048):
     """Pairs with r < rc and triplets with all three legs < rc, from the
     batch's own neighbor lists (counted on the GPU)."""
@@ -210,9 +218,7 @@ def batched_bp(steps, warmup):
             "flop_per_launch": flops,
             "achieved_tflops": flops / k_s / 1e12,
             "frac_fp32_peak": flops / k_s / FP32_PEAK,
-            "launch_shape": {"chosen": {"waves_per_workgroup": chosen[0], "weights_in_lds": chosen[1]},
-                             "candidates_us": [{"waves_per_workgroup": w, "weights_in_lds": s, "kernel_us": t * 1e3}
-                                               for (w, s, t) in rows]},
+            "launch_shape": launch_shape(chosen, rows),
         }
         del db
         return out
@@ -362,9 +368,7 @@ def main():
     if not args.no_autotune:
         # the atom kernel's launch shape by measurement (aenet_amd_batch_autotune)
         chosen, rows = db.autotune(forces=True, repeats=10)
-        tune = {"chosen": {"waves_per_workgroup": chosen[0], "weights_in_lds": chosen[1]},
-                "candidates_us": [{"waves_per_workgroup": w, "weights_in_lds": s, "kernel_us": t * 1e3}
-                                  for (w, s, t) in rows]}
+        tune = launch_shape(chosen, rows)
     # eager launches (five kernels per step)
     for _ in range(args.warmup):
         db.eval(check=False)

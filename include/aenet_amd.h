@@ -106,20 +106,33 @@ void aenet_amd_batch_search_time(aenet_amd_batch *batch, double *ms, int *count,
                                  int *stat);
 
 /* Autotuner of the atom kernel's launch shape for this batch and model
- * (NULL: the potentials of aenet_load_potential): every feasible pair
+ * (NULL: the potentials of aenet_load_potential): every feasible triple
  * (wavefronts per workgroup: 1, 2, 4, .. 16 for Chebyshev, .. 4 for
  * Behler-Parrinello; network weights staged in LDS or read through the
- * caches) is timed with HIP events over `repeats` full evaluations of the
- * given device coordinates and types (with forces if forces != 0), and the
- * fastest is used by the batch's later evaluations with that model.
- * Without it the launch shape comes from an occupancy model.  results (may
- * be NULL, room for 16 rows): per candidate (waves, staged, average ms);
- * *ncand = their number; chosen[0..1] = the kept (waves, staged).
+ * caches; network per wavefront or, for Chebyshev potentials whose species
+ * share one network architecture, the networks of a workgroup's atoms as
+ * one matrix-core tile) is timed with HIP events over `repeats` full
+ * evaluations of the given device coordinates and types (with forces if
+ * forces != 0), and the fastest is used by the batch's later evaluations
+ * with that model.  Without it the launch shape comes from an occupancy
+ * model, with the network per wavefront.  results (may be NULL, room for 32
+ * rows): per candidate (waves, staged, tile, average ms); *ncand = their
+ * number; chosen[0..2] = the kept (waves, staged, tile).
  * Synchronous.  No counterpart in the reference. */
 void aenet_amd_batch_autotune(aenet_amd_batch *batch, const aenet_amd_model *model,
                               const double *d_coords, const int *d_types, int forces,
-                              int repeats, double results[], int *ncand, int chosen[2],
+                              int repeats, double results[], int *ncand, int chosen[3],
                               void *stream, int *stat);
+
+/* Sets the atom kernel's launch shape for this batch and model as the
+ * autotuner would keep it: waves per workgroup (a power of two up to 16 for
+ * Chebyshev, 4 for Behler-Parrinello), weights staged in LDS (0/1), tile
+ * network (0/1; Chebyshev, one architecture for every species).  waves = 0
+ * returns to the occupancy model.  AENET_AMD_ERR_ARG_VALUE for a shape out
+ * of range or one that does not fit the device.  No counterpart in the
+ * reference. */
+void aenet_amd_batch_set_launch(aenet_amd_batch *batch, const aenet_amd_model *model,
+                                int waves, int staged, int tile, int *stat);
 
 /* Bit-exact neighbor lists (lcl_nbdist_cart, src/ext/lclist.f90:520-677)
  * of every atom of the batch, device pointers, capacity cap per atom:

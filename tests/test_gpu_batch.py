@@ -149,10 +149,14 @@ def test_graph_capture_replays_match_eager(structures, device):
 
 
 def test_autotune_keeps_results(device):
-    """The measured launch shape changes nothing but the time: per-atom
-    energies are bitwise those of the occupancy model's shape (one wavefront
-    computes each atom either way), forces equal up to the order of the FP64
-    atomic sums; every candidate was timed and the kept one is the fastest."""
+    """The measured launch shape changes nothing but the time: with the
+    network per wavefront, per-atom energies are bitwise those of the
+    occupancy model's shape (one wavefront computes each atom either way),
+    forces equal up to the order of the FP64 atomic sums; with the tile
+    network (its FP32 products summed in another order) both within the
+    per-atom bars.  Every candidate was timed (the tile network among them
+    where the species share one architecture) and the kept one is the
+    fastest."""
     import torch
     from aenet_amd import synth
     from aenet_amd.core import ANNPotentials
@@ -168,13 +172,75 @@ def test_autotune_keeps_results(device):
             E0, F0 = E0.clone(), F0.clone()
             chosen, rows = db.autotune(forces=True, repeats=2)
             print("chosen", chosen, "rows", rows)
-            assert rows and chosen in [(w, s) for (w, s, _) in rows]
-            assert min(rows, key=lambda r: r[2])[:2] == chosen
+            assert rows and chosen in [(w, s, t) for (w, s, t, _) in rows]
+            assert min(rows, key=lambda r: r[3])[:3] == chosen
+            assert any(t == 1 for (_, _, t, _) in rows) == (pots == pot_files("alloy", ["A", "B"]))
             _, _, E1, F1 = db.eval()
-            assert torch.equal(E0, E1)
-            assert (F1 - F0).abs().max().item() < 1e-10
+            if chosen[2] == 0:
+                assert torch.equal(E0, E1)
+                assert (F1 - F0).abs().max().item() < 1e-10
+            else:
+                assert (E1 - E0).abs().max().item() < 2 * E_TOL
+                assert (F1 - F0).abs().max().item() < 2 * F_TOL
         finally:
             ann.release()
+
+
+def test_launch_shapes_and_tile_network(device):
+    """Every launch shape of the Chebyshev atom kernel set explicitly
+    (aenet_amd_batch_set_launch), with the network per wavefront and as one
+    matrix-core tile per workgroup round, against the occupancy model's
+    shape: per-wavefront network bitwise (energies) and to FP64 summation
+    order (forces); tile network within the per-atom bars (FP32 products
+    summed in another order).  Structures: 500 atoms (the last workgroup
+    partly filled) and five 4000-atom cells (several rounds per workgroup).
+    Before each checked evaluation the same shape evaluates displaced
+    coordinates, so that values an evaluation fails to write would show as
+    the other coordinates' values."""
+    import torch
+    from aenet_amd import synth
+    from aenet_amd.core import ANNPotentials
+    from aenet_amd.device import DeviceBatch
+    ann = ANNPotentials(pot_files("alloy", ["A", "B"]))
+    try:
+        for structs in ([synth.binary_fcc(3.8, (5, 5, 5), 0.08, 43, 44)], synth.alloy_replicas(5)):
+            db = DeviceBatch(structs, device=device)
+            _, _, E0, F0 = db.eval()
+            E0, F0 = E0.clone(), F0.clone()
+            other = db.coords + 0.05 * torch.sin(torch.arange(db.coords.numel(), device=db.coords.device,
+                                                              dtype=torch.float64)).reshape(db.coords.shape)
+            for tile in (0, 1):
+                for waves in (1, 2, 4, 8, 16):
+                    for staged in (0, 1):
+                        print("shape", len(structs), waves, staged, tile, flush=True)
+                        db.set_launch(waves, staged, tile)
+                        _, _, Eo, _ = db.eval(coords=other)
+                        assert (Eo - E0).abs().max().item() > 1e-3
+                        _, _, E1, F1 = db.eval()
+                        if tile == 0:
+                            assert torch.equal(E0, E1)
+                            assert (F1 - F0).abs().max().item() < 1e-10
+                        else:
+                            assert (E1 - E0).abs().max().item() < 2 * E_TOL
+                            assert (F1 - F0).abs().max().item() < 2 * F_TOL
+            db.set_launch(0)
+            _, _, E2, _ = db.eval()
+            assert torch.equal(E0, E2)
+            with pytest.raises(Exception):
+                db.set_launch(3, 0, 0)
+            del db
+    finally:
+        ann.release()
+    # no tile network for Behler-Parrinello potentials
+    ann = ANNPotentials(pot_files("bp3", NAMES))
+    try:
+        db = DeviceBatch(synth.batch_uneven(8), device=device)
+        with pytest.raises(Exception):
+            db.set_launch(4, 0, 1)
+        db.set_launch(4, 0, 0)
+        del db
+    finally:
+        ann.release()
 
 
 def test_isolated_structure_leaving_its_box_is_replanned(device):
