@@ -167,6 +167,7 @@ struct BatchDev {
   const int *self_t;          // self-image ranks
   const double *self_d;       // self-image distances
   int *flags;
+  int *fmark;                 // [natoms] a force contribution outside the fixed-point range (see scatter_forces)
 };
 
 // compact neighbor lists written by the search kernel for the atom kernels

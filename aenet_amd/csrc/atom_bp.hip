@@ -25,7 +25,8 @@
 //   5. forces     : the same traversal again with dE/dG: per pair / triplet
 //                   the function sums collapse to dE/dr_ij, dE/dr_ik,
 //                   dE/dr_jk and dE/dcos, which are turned into per-neighbor
-//                   gradients in LDS, then FP64 atomics into F
+//                   gradients in LDS, then fixed-point (64-bit integer) atomics
+//                   into F
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -406,7 +407,7 @@ struct AngForceOp {
   }
 };
 
-// MODE 0: structures (compact lists, FP64 atomic force scatter)
+// MODE 0: structures (compact lists, fixed-point atomic force scatter)
 // MODE 1: explicit environments (per-neighbor forces out)
 template <int MODE, bool FORCES>
 __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, BatchDev b, NbDev nbl, OutDev o,
@@ -673,33 +674,28 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
     }
     wave_sync();
     // F_j -= dE_i/dR_j ; F_i += sum_j dE_i/dR_j  (src/aenet.f90:471-477)
-    double own[3] = {0.0, 0.0, 0.0};
-    const int *Jl = MODE == 0 ? nbl.j + (size_t)g * nbl.cap : nullptr;  // neighbor ids from the list
-    const int e0 = MODE == 0 ? 0 : env.env_start[g];
-    for (int q = lane; q < n; q += WAVE) {
-      double gx = S.fs[3 * q], gy = S.fs[3 * q + 1], gz = S.fs[3 * q + 2];
-      own[0] += gx;
-      own[1] += gy;
-      own[2] += gz;
-      if (MODE == 0) {
-        int j = Jl[q];
-        unsafeAtomicAdd(&o.F[3 * (size_t)j], -gx);
-        unsafeAtomicAdd(&o.F[3 * (size_t)j + 1], -gy);
-        unsafeAtomicAdd(&o.F[3 * (size_t)j + 2], -gz);
-      } else {
+    if constexpr (MODE == 0) {
+      const int *Jl = nbl.j + (size_t)g * nbl.cap;  // neighbor ids from the list
+      scatter_forces(o.F, b.fmark, g, n, Jl, [&](int q, double (&c)[3]) {
+        c[0] = S.fs[3 * q];
+        c[1] = S.fs[3 * q + 1];
+        c[2] = S.fs[3 * q + 2];
+      });
+    } else {
+      double own[3] = {0.0, 0.0, 0.0};
+      const int e0 = env.env_start[g];
+      for (int q = lane; q < n; q += WAVE) {
+        double gx = S.fs[3 * q], gy = S.fs[3 * q + 1], gz = S.fs[3 * q + 2];
+        own[0] += gx;
+        own[1] += gy;
+        own[2] += gz;
         size_t e = (size_t)(e0 + q);
         env.dF_j[3 * e] = -gx;
         env.dF_j[3 * e + 1] = -gy;
         env.dF_j[3 * e + 2] = -gz;
       }
-    }
-    for (int x = 0; x < 3; ++x) own[x] = wave_sum(own[x]);
-    if (lane == 0) {
-      if (MODE == 0) {
-        unsafeAtomicAdd(&o.F[3 * (size_t)g], own[0]);
-        unsafeAtomicAdd(&o.F[3 * (size_t)g + 1], own[1]);
-        unsafeAtomicAdd(&o.F[3 * (size_t)g + 2], own[2]);
-      } else {
+      for (int x = 0; x < 3; ++x) own[x] = wave_sum(own[x]);
+      if (lane == 0) {
         env.dF_i[3 * g] = own[0];
         env.dF_i[3 * g + 1] = own[1];
         env.dF_i[3 * g + 2] = own[2];

@@ -21,7 +21,9 @@
 //                   workgroup (or read through the caches)
 //   5. forces     : F_j -= E_scale dy/dG dG/dR_j (src/aenet.f90:471-477):
 //                   pair and triplet derivatives contracted with dE/dG in
-//                   FP32, per-neighbor sums in LDS, FP64 atomics into F
+//                   FP32, per-neighbor sums in LDS, fixed-point (64-bit integer)
+//                   atomics into F, so that forces do not depend on the
+//                   order of the sums
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -247,7 +249,7 @@ __device__ inline void network_rows(WPtr W, float *regions, const ChebLaunch &P,
   }
 }
 
-// MODE 0: structures (compact lists, FP64 atomic force scatter)
+// MODE 0: structures (compact lists, fixed-point atomic force scatter)
 // MODE 1: explicit environments (per-neighbor forces out)
 // TILE: the network of a round's atoms as one matrix tile (MODE 0)
 template <int RN, int AN, int MODE, bool FORCES, bool TILE>
@@ -653,31 +655,26 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
     }
     wave_sync();
     // F_j -= dE_i/dR_j ; F_i += sum_j dE_i/dR_j
-    double own[3] = {0.0, 0.0, 0.0};
-    for (int q = lane; q < n; q += WAVE) {
-      double gx = fs[3 * q], gy = fs[3 * q + 1], gz = fs[3 * q + 2];
-      own[0] += gx;
-      own[1] += gy;
-      own[2] += gz;
-      if (MODE == 0) {
-        int j = S.j[q];
-        unsafeAtomicAdd(&o.F[3 * (size_t)j], -gx);
-        unsafeAtomicAdd(&o.F[3 * (size_t)j + 1], -gy);
-        unsafeAtomicAdd(&o.F[3 * (size_t)j + 2], -gz);
-      } else {
+    if constexpr (MODE == 0) {
+      scatter_forces(o.F, b.fmark, g, n, S.j, [&](int q, double (&c)[3]) {
+        c[0] = fs[3 * q];
+        c[1] = fs[3 * q + 1];
+        c[2] = fs[3 * q + 2];
+      });
+    } else {
+      double own[3] = {0.0, 0.0, 0.0};
+      for (int q = lane; q < n; q += WAVE) {
+        double gx = fs[3 * q], gy = fs[3 * q + 1], gz = fs[3 * q + 2];
+        own[0] += gx;
+        own[1] += gy;
+        own[2] += gz;
         size_t e = (size_t)S.j[q];
         env.dF_j[3 * e] = -gx;
         env.dF_j[3 * e + 1] = -gy;
         env.dF_j[3 * e + 2] = -gz;
       }
-    }
-    for (int x = 0; x < 3; ++x) own[x] = wave_sum(own[x]);
-    if (lane == 0) {
-      if (MODE == 0) {
-        unsafeAtomicAdd(&o.F[3 * (size_t)g], own[0]);
-        unsafeAtomicAdd(&o.F[3 * (size_t)g + 1], own[1]);
-        unsafeAtomicAdd(&o.F[3 * (size_t)g + 2], own[2]);
-      } else {
+      for (int x = 0; x < 3; ++x) own[x] = wave_sum(own[x]);
+      if (lane == 0) {
         env.dF_i[3 * g] = own[0];
         env.dF_i[3 * g + 1] = own[1];
         env.dF_i[3 * g + 2] = own[2];

@@ -180,6 +180,7 @@ int Batch::allocate(std::string &err) {
   work_cap_ = p.natoms / CS_WAVES + p.ncells + 1;
   bad |= cell_work_.alloc(4 * (size_t)work_cap_) != 0;
   bad |= scratch_E_.alloc(p.natoms) != 0;
+  bad |= fmark_.alloc(p.natoms) != 0;
   if (bad) {
     err = "device allocation failed";
     return AENET_STATUS_ERR_MALLOC;
@@ -209,6 +210,7 @@ int Batch::allocate(std::string &err) {
   // hipMalloc does not clear memory: the error word and the cell counters
   // must start at zero (each step leaves the counters zeroed again)
   launch_fill_i32(flags_.p, 0, 1, s);
+  launch_fill_i32(fmark_.p, 0, p.natoms, s);
   launch_fill_i32(cell_count_.p, 0, p.ncells, s);
   launch_fill_i32(cell_fill_.p, 0, p.ncells, s);
   // the (cell, image) combinations depend on the lattice only
@@ -283,6 +285,7 @@ BatchDev Batch::view(const double *d_coords, const int *d_types) const {
   b.self_t = self_t_.p;
   b.self_d = self_d_.p;
   b.flags = flags_.p;
+  b.fmark = fmark_.p;
   return b;
 }
 
@@ -323,10 +326,9 @@ int Batch::eval(const DeviceModel &dm, const double *d_coords, const int *d_type
     ev_.push_back(e0);
     ev_.push_back(e1);
   }
-  if (d_E_struct) {
-    // E_struct = [E_tot(0..ns-1), E_coh(0..ns-1)]
-    launch_struct_energy(b, md, o.E_atom, d_E_struct, d_E_struct + b.nstruct, stream);
-  }
+  // E_struct = [E_tot(0..ns-1), E_coh(0..ns-1)]; the forces into FP64
+  launch_struct_energy(b, md, o.E_atom, d_E_struct, d_E_struct ? d_E_struct + b.nstruct : nullptr, d_F, 1.0,
+                       stream);
   return hipGetLastError() == hipSuccess ? AENET_STATUS_OK : AENET_STATUS_ERR_DEVICE;
 }
 
@@ -350,7 +352,7 @@ int Batch::vjp(const DeviceModel &dm, const double *d_coords, const int *d_types
   else
     launch_atom_bp(0, true, z, md, b, nb, o, e, stream);
   // the kernels accumulate forces, -gradient
-  launch_scale_f64(d_gR, -1.0, 3 * (long)b.natoms, stream);
+  launch_struct_energy(b, md, nullptr, nullptr, nullptr, d_gR, -1.0, stream);
   if (d_gW) {
     const size_t need = (size_t)b.natoms * dm.nsf_max;
     if (scratch_G_.n < need && scratch_G_.alloc(need)) return AENET_STATUS_ERR_MALLOC;

@@ -30,8 +30,10 @@ void launch_search(const BatchDev &b, const NbDev &nb, hipStream_t s, const int 
                    const int *nactive = nullptr, int max_active = 0);
 // bit-exact FP64 neighbor lists
 void launch_neighbor_lists(const BatchDev &b, const OutDev &o, hipStream_t s);
+// per-structure energies (E_tot null: none) and, with F, the fixed-point
+// forces of the atom kernels turned into FP64 times fscale
 void launch_struct_energy(const BatchDev &b, const ModelDev &md, const double *E_atom, double *E_tot,
-                          double *E_coh, hipStream_t s);
+                          double *E_coh, double *F, double fscale, hipStream_t s);
 
 // atom_cheb.hip: mode 0 = structures, 1 = explicit environments
 // properties of the current device the launchers size their grids by

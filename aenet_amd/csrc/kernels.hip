@@ -398,14 +398,33 @@ __global__ __launch_bounds__(256) void k_combos(BatchDev b, int *combo_count, in
   }
 }
 
-// per-structure energy sums in FP64 (fixed reduction order: thread t sums
-// atoms t, t+256, ... in turn, then a tree over the threads); the loads of
-// SE_UNROLL atoms issue before their sums
+// The end of an evaluation, one launch:
+// - workgroups 0 .. nsum-1: per-structure energy sums in FP64 (fixed
+//   reduction order: thread t sums atoms t, t+256, ... in turn, then a tree
+//   over the threads); the loads of SE_UNROLL atoms issue before their sums;
+// - the workgroups after them, with F: the fixed-point force sums of the
+//   atom kernels (scatter_forces) turned into FP64 times fscale, a thread per
+//   atom; an atom marked out of range gets NaN and its mark is cleared.
 constexpr int SE_UNROLL = 8;
 __global__ __launch_bounds__(256) void k_struct_energy(BatchDev b, ModelDev md, const double *E_atom,
-                                                       double *E_tot, double *E_coh) {
+                                                       double *E_tot, double *E_coh, int nsum, double *F,
+                                                       double fscale) {
   __shared__ double st[256], sc[256];
   __shared__ double e_free[AENET_AMD_MAX_TYPES];
+  if ((int)blockIdx.x >= nsum) {
+    const int g = (blockIdx.x - nsum) * 256 + threadIdx.x;
+    if (g < b.natoms) {
+      long long *Fi = reinterpret_cast<long long *>(F) + 3 * (size_t)g;
+      const long long v0 = Fi[0], v1 = Fi[1], v2 = Fi[2];
+      const bool bad = b.fmark[g] != 0;
+      const double nan = __builtin_nan("");
+      F[3 * (size_t)g] = bad ? nan : (double)v0 * fscale;
+      F[3 * (size_t)g + 1] = bad ? nan : (double)v1 * fscale;
+      F[3 * (size_t)g + 2] = bad ? nan : (double)v2 * fscale;
+      if (bad) b.fmark[g] = 0;
+    }
+    return;
+  }
   const int s = blockIdx.x;
   const StructDev *S = b.st + s;
   const int n = S->natoms, a0 = S->atom0;
@@ -875,9 +894,12 @@ void launch_neighbor_lists(const BatchDev &b, const OutDev &o, hipStream_t s) {
 }
 
 void launch_struct_energy(const BatchDev &b, const ModelDev &md, const double *E_atom, double *E_tot,
-                          double *E_coh, hipStream_t s) {
-  if (b.nstruct > 0)
-    hipLaunchKernelGGL(k_struct_energy, dim3(b.nstruct), dim3(256), 0, s, b, md, E_atom, E_tot, E_coh);
+                          double *E_coh, double *F, double fscale, hipStream_t s) {
+  const int nsum = E_tot ? b.nstruct : 0;
+  const int nconv = F ? nblk(b.natoms, 256) : 0;
+  if (nsum + nconv > 0)
+    hipLaunchKernelGGL(k_struct_energy, dim3(nsum + nconv), dim3(256), 0, s, b, md, E_atom, E_tot, E_coh, nsum, F,
+                       fscale * FORCE_FIX_UNIT);
 }
 
 }  // namespace aenet_amd

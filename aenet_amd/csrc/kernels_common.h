@@ -80,6 +80,14 @@ __device__ __forceinline__ void half_swap(double &a, double &b) {
 }
 
 template <int O>
+__device__ __forceinline__ void half_swap(long long &a, long long &b) {
+  double x = __longlong_as_double(a), y = __longlong_as_double(b);
+  half_swap<O>(x, y);  // moves the bits only
+  a = __double_as_longlong(x);
+  b = __double_as_longlong(y);
+}
+
+template <int O>
 __device__ __forceinline__ unsigned partner_u32(unsigned x) {
   static_assert(O == 8 || O == 4 || O == 2 || O == 1, "DPP within a row");
   constexpr int ctrl = O == 8 ? 0x128 : O == 4 ? 0x141 : O == 2 ? 0x4E : 0xB1;
@@ -96,6 +104,11 @@ __device__ __forceinline__ double partner(double x) {
   const unsigned long long u = (unsigned long long)__double_as_longlong(x);
   const unsigned lo = partner_u32<O>((unsigned)u), hi = partner_u32<O>((unsigned)(u >> 32));
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <int O>
+__device__ __forceinline__ long long partner(long long x) {
+  return __double_as_longlong(partner<O>(__longlong_as_double(x)));  // moves the bits only
 }
 
 // sum over the lanes of the butterfly levels O, O/2, .., 1 (every lane
@@ -116,6 +129,76 @@ __device__ __forceinline__ T butterfly_sum(T v) {
 __device__ __forceinline__ float wave_sum(float v) { return butterfly_sum<32>(v); }
 
 __device__ __forceinline__ double wave_sum(double v) { return butterfly_sum<32>(v); }
+
+__device__ __forceinline__ long long wave_sum(long long v) { return butterfly_sum<32>(v); }
+
+// Forces of structures are summed as 64-bit fixed-point integers in units
+// of 2^-32 eV/A: integer addition is associative, so the sum of the
+// contributions of all atoms to F_j does not depend on the order in which
+// the atomics arrive, and the forces are bitwise the same from run to run.
+// The force array is reinterpreted during the sum (zero bits are zero in
+// both forms) and turned into FP64 by k_struct_energy.  A contribution of
+// 2^18 eV/A or more, or not finite, is not added (the sum over up to 512
+// neighbors stays below 2^27 eV/A, inside the 2^31 eV/A of the format), and
+// the atom is marked instead, which makes its force NaN.
+constexpr double FORCE_FIX_SCALE = 4294967296.0;        // 2^32
+constexpr double FORCE_FIX_UNIT = 1.0 / 4294967296.0;   // 2^-32
+constexpr double FORCE_FIX_MAX = 262144.0;              // 2^18
+
+// the fixed-point form of a contribution c; false when out of range.
+// |c| 2^32 < 2^50: adding 1.5 2^52 rounds it to an integer (to nearest,
+// ties to even) held in the low mantissa bits, and subtracting the bits of
+// 1.5 2^52 leaves that integer in two's complement
+__device__ __forceinline__ bool force_fix(const double (&c)[3], long long (&v)[3]) {
+  if (!(fabs(c[0]) < FORCE_FIX_MAX && fabs(c[1]) < FORCE_FIX_MAX && fabs(c[2]) < FORCE_FIX_MAX)) return false;
+  constexpr double magic = 6755399441055744.0;  // 1.5 2^52
+#pragma unroll
+  for (int x = 0; x < 3; ++x)
+    v[x] = __double_as_longlong(c[x] * FORCE_FIX_SCALE + magic) - __double_as_longlong(magic);
+  return true;
+}
+
+__device__ __forceinline__ void force_add_fixed(double *F, int atom, const long long (&v)[3]) {
+  unsigned long long *Fi = reinterpret_cast<unsigned long long *>(F) + 3 * (size_t)atom;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) atomicAdd(Fi + x, (unsigned long long)v[x]);
+}
+
+// The scatter of one central atom i over its n neighbors: per neighbor q
+// the FP32 gradient dE_i/dR_j (grad(q) -> double[3]) goes to F_j with a
+// minus sign, and F_i receives minus the integer sum of exactly those
+// fixed-point values, so the forces of a structure sum to zero exactly in
+// the fixed-point units (translation invariance).  A contribution out of
+// range marks j and i.
+template <class Grad>
+__device__ inline void scatter_forces(double *F, int *fmark, int i, int n, const int *J, Grad grad) {
+  long long own[3] = {0, 0, 0};
+  bool bad = false;
+  for (int q = lane_id(); q < n; q += WAVE) {
+    double c[3];
+    grad(q, c);
+    const double m[3] = {-c[0], -c[1], -c[2]};
+    long long v[3];
+    const int j = J[q];
+    if (force_fix(m, v)) {
+      force_add_fixed(F, j, v);
+      own[0] -= v[0];
+      own[1] -= v[1];
+      own[2] -= v[2];
+    } else {
+      fmark[j] = 1;
+      bad = true;
+    }
+  }
+  for (int x = 0; x < 3; ++x) own[x] = wave_sum(own[x]);
+  bad = wave_ballot(bad) != 0;
+  if (lane_id() == 0) {
+    if (bad)
+      fmark[i] = 1;
+    else
+      force_add_fixed(F, i, own);
+  }
+}
 
 // inclusive prefix sum over the wavefront
 __device__ __forceinline__ int wave_scan(int x) {
