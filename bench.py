@@ -5,7 +5,7 @@ One step = one full pass of the prediction path over the structure held in
 HBM: Cartesian -> lattice coordinates, cell binning, ordered FP64 neighbor
 search, Chebyshev descriptors (radial 18 / 6.5 A, angular 6 / 4.5 A,
 typespin), per-species 52-30-30-1 network forward and backward, forces
-(FP64 accumulation) and per-structure energy sums.
+(64-bit fixed-point sums, converted to FP64) and per-structure energy sums.
 
 Prints one JSON line.  ``roofline`` is for the fused atom kernel, timed with
 HIP events on its own stream; ``cpu_baseline`` is the reference library
@@ -241,7 +241,8 @@ def mc_throughput(steps, burn):
     cells = [synth.mc_cell(100 + c) for c in range(512)]
     out = {"workload": "mc512x256: 512 chains x 256 atoms (4x4x4 FCC A/B, a=3.61 A), 600 K, 50% swaps, "
                        "50% displacements U(+-0.075 A)^3; teacher Chebyshev 16/6.0 + 4/4.0 typespin 44-20-20-1; "
-                       "first stage radial-only Chebyshev 8/5.0 typespin 20-8-1 trained on teacher MC data",
+                       "first stage radial-only Chebyshev 8/5.0 typespin 20-8-1 trained on teacher MC data (atomic energies and "
+                       "forces, then the energy changes of proposals)",
            "steps": steps, "burn_in": burn}
     for name, first, delayed in (("plain", None, False), ("delayed", student, True)):
         mc = MonteCarlo(teacher, cells, first=first, seed=7)
@@ -319,6 +320,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched Behler-Parrinello measurement")
     ap.add_argument("--no-mc", action="store_true", help="skip the Monte Carlo measurement")
+    ap.add_argument("--mc-steps", type=int, default=3000, help="Monte Carlo steps per chain after the burn-in")
+    ap.add_argument("--mc-burn", type=int, default=1000)
+    ap.add_argument("--mc-only", action="store_true", help="only the Monte Carlo measurement (its own JSON line)")
     ap.add_argument("--no-per-atom", action="store_true", help="skip the per-atom C-ABI measurement")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the occupancy model's launch shape instead of the measured one")
@@ -340,6 +344,9 @@ def main():
         print("bench.py measures one GPU only: the prediction path runs on a single MI355X and this build "
               "has no multi-GPU mode (--gpus %d requested)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.mc_only:
+        print(json.dumps({"mc": mc_throughput(args.mc_steps, args.mc_burn)}))
+        return
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -501,7 +508,7 @@ def main():
     if not args.no_batch:
         out["batched_bp"] = batched_bp(max(5, min(args.steps, 20)), 2)
     if not args.no_mc:
-        out["mc"] = mc_throughput(3000, 1000)
+        out["mc"] = mc_throughput(args.mc_steps, args.mc_burn)
     print(json.dumps(out))
 
 
