@@ -293,6 +293,7 @@ def main():
     ap.add_argument("--snapshots", type=int, default=24)
     ap.add_argument("--between", type=int, default=200)
     ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--batch", type=int, default=128, help="cells per Adam step of the energy/force fit")
     ap.add_argument("--move-snapshots", type=int, default=48,
                     help="snapshots of proposals (one per chain each) for the energy-change stage; 0: skip it")
     ap.add_argument("--move-epochs", type=int, default=60)
@@ -321,7 +322,7 @@ def main():
         data = collect(mc, teacher, args.snapshots, args.between, dev)
         tr, va = split(data, 4 * args.chains)  # the last 4 snapshots are held out
         e0, f0 = errors(student, va)
-        train(student, tr, epochs=args.epochs, log=print)
+        train(student, tr, epochs=args.epochs, cells_per_step=args.batch, log=print)
         e1, f1 = errors(student, va)
         print("train %.1f s; held-out RMSE: atomic energy %.4f -> %.4f eV, force %.4f -> %.4f eV/A"
               % (time.perf_counter() - t0, e0, e1, f0, f1), flush=True)
