@@ -324,6 +324,9 @@ def main():
     ap.add_argument("--mc-burn", type=int, default=1000)
     ap.add_argument("--mc-only", action="store_true", help="only the Monte Carlo measurement (its own JSON line)")
     ap.add_argument("--no-per-atom", action="store_true", help="skip the per-atom C-ABI measurement")
+    ap.add_argument("--launch", default=None, metavar="W,S,T",
+                    help="atom-kernel launch shape instead of the tuner's: wavefronts per workgroup, weights "
+                         "in LDS (0/1), tile network (0/1) (aenet_amd_batch_set_launch)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the occupancy model's launch shape instead of the measured one")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -372,7 +375,11 @@ def main():
     db = DeviceBatch(structs)
     natoms = db.natoms
     tune = None
-    if not args.no_autotune:
+    if args.launch:
+        w, st, tl = (int(v) for v in args.launch.split(","))
+        db.set_launch(w, st, tl)
+        tune = {"chosen": {"waves_per_workgroup": w, "weights_in_lds": st, "tile_network": tl}, "set": True}
+    elif not args.no_autotune:
         # the atom kernel's launch shape by measurement (aenet_amd_batch_autotune)
         chosen, rows = db.autotune(forces=True, repeats=10)
         tune = launch_shape(chosen, rows)
@@ -438,9 +445,12 @@ def main():
     achieved = flops / k_avg_s
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_atom_cheb_alloy4000_traffic.json")
+    tile = bool(tune and tune["chosen"].get("tile_network"))
+    kname = "k_atom_cheb<19,7,0,true,%s>" % ("true" if tile else "false")
     if args.replicas == 1 and os.path.exists(tpath):
-        t = json.load(open(tpath))["kernels"]["k_atom_cheb<19,7,0,true>"]
-        traffic = t["read_bytes"] + t["write_bytes"]
+        t = json.load(open(tpath))["kernels"].get(kname)
+        if t:
+            traffic = t["read_bytes"] + t["write_bytes"]
 
     out = {
         "metric": METRIC,
@@ -466,7 +476,7 @@ def main():
         "roofline": {
             "bound": "valu",
             "pipe": "FP32 VALU: the atom kernel issues no MFMA; peak = MI355X FP32 vector 157.3 TFLOP/s",
-            "kernel": "k_atom_cheb<19,7>",
+            "kernel": kname,
             "achieved": achieved / 1e12,
             "peak": FP32_PEAK / 1e12,
             "unit": "TFLOP/s",
