@@ -611,10 +611,9 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
       cap_[k] = k < aN ? ca1[k] : 0.0f;
       cas_[k] = (multi && k < aN) ? ca2[k] : 0.0f;
     }
-    for (int p = lane; p < ntrip; p += WAVE) {
-      int a, c;
-      tri_decode_diag(p, na, a, c);
-      int qj = S.ang[a], qk = S.ang[c];
+    // gradients of triplet (angular neighbors a, c) with respect to R_j and R_k
+    auto trip_grad = [&](int a, int c, float (&gj)[3], float (&gk)[3]) {
+      const int qj = S.ang[a], qk = S.ang[c];
       const float4 Aj = ang_get(a, qj), Ak = ang_get(c, qk);
       const float ij = Aj.x, ik = Ak.x;
       float Rj[3] = {S.R[3 * qj], S.R[3 * qj + 1], S.R[3 * qj + 2]};
@@ -647,10 +646,73 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
       float cj = cosv * ij * ij, ck = cosv * ik * ik;
 #pragma unroll
       for (int x = 0; x < 3; ++x) {
-        float gj = aj * Rj[x] + wB * (Rk[x] * ijk - cj * Rj[x]);
-        float gk = ak * Rk[x] + wB * (Rj[x] * ijk - ck * Rk[x]);
-        atomicAdd(&fs[3 * qj + x], gj);
-        atomicAdd(&fs[3 * qk + x], gk);
+        gj[x] = aj * Rj[x] + wB * (Rk[x] * ijk - cj * Rj[x]);
+        gk[x] = ak * Rk[x] + wB * (Rj[x] * ijk - ck * Rk[x]);
+      }
+    };
+    // Schedule of the angular force pass.  With the per-neighbor table
+    // (na <= AUXN), the lanes form G = 64 / na groups of na lanes and lane
+    // (group h, a) owns angular neighbor a: in round d (1 .. na/2, the rounds
+    // dealt round-robin to the groups) it takes the triplet (a, a + d mod na)
+    // -- every unordered pair once, the half round d = na/2 only for a < d --
+    // keeps its own gradient in registers and hands the partner's to the
+    // partner's owner in its group with one cross-lane read per component.
+    // The groups' sums are added per neighbor at the end: no atomics.  When
+    // that takes more than twice the rounds of 64 triplets at a time, the
+    // triplets go over the lanes as in the value pass and add to the
+    // per-neighbor sums with LDS atomics.
+    const int rounds = na / 2;
+    const int groups = na >= 2 ? WAVE / na : 1;
+    const int rot_iters = na >= 2 ? (rounds + groups - 1) / groups : 0;
+    if (use_aux && na >= 2 && rot_iters <= 2 * ((ntrip + WAVE - 1) / WAVE)) {
+      const int h = lane / na, a = lane - h * na;
+      float acc[3] = {0.0f, 0.0f, 0.0f};
+      for (int it = 0; it < rot_iters; ++it) {
+        const int d = it * groups + h + 1;
+        const bool in_round = h < groups && d <= rounds;
+        const bool act = in_round && !(2 * d == na && a >= d);
+        float gk[3] = {0.0f, 0.0f, 0.0f};
+        if (act) {
+          int c = a + d;
+          if (c >= na) c -= na;
+          float gj[3];
+          trip_grad(a, c, gj, gk);
+#pragma unroll
+          for (int x = 0; x < 3; ++x) acc[x] += gj[x];
+        }
+        // lane (h, a) takes the gradient of lane (h, a - d), whose partner is a
+        int sa = a - d;
+        if (sa < 0) sa += na;
+        const int src = in_round ? h * na + sa : lane;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) acc[x] += __shfl(gk[x], src, WAVE);
+      }
+      float tot[3] = {acc[0], acc[1], acc[2]};
+      for (int hh = 1; hh < groups; ++hh) {
+        const int src = h == 0 ? hh * na + a : lane;
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          const float t = __shfl(acc[x], src, WAVE);
+          if (h == 0) tot[x] += t;
+        }
+      }
+      if (h == 0) {  // lanes a < na: distinct neighbor slots
+        const int q = S.ang[a];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) fs[3 * q + x] += tot[x];
+      }
+    } else {
+      for (int p = lane; p < ntrip; p += WAVE) {
+        int a, c;
+        tri_decode_diag(p, na, a, c);
+        float gj[3], gk[3];
+        trip_grad(a, c, gj, gk);
+        const int qj = S.ang[a], qk = S.ang[c];
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+          atomicAdd(&fs[3 * qj + x], gj[x]);
+          atomicAdd(&fs[3 * qk + x], gk[x]);
+        }
       }
     }
     wave_sync();
