@@ -294,6 +294,8 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
     wave_sync();
     // phases 1-3; false for an atom of an unknown species
     int ti = 0, n = 0, na = 0;
+    int lane_f = -1;       // the species feature this lane holds after the reduce-scatter (-1: none)
+    float lane_x = 0.0f;   // its normalized value
     const bool front_ok = live && [&]() -> bool {
       // ------------------------------------------------------------ 1
       int spw;
@@ -487,6 +489,8 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
           const double av = P.w_in_lds ? ((lds_f64)lnorm)[sp.norm_off + f] : md.norm_avg[sp.norm_off + f];
           double Gs = sc * (G - av);
           S.net.gp[f] = (float)Gs;
+          lane_f = f;
+          lane_x = (float)Gs;
           S.net.coef[f] = (float)sc;  // for the network's dE/dG
           if (o.G) o.G[(size_t)g * o.G_stride + f] = o.G_scaled ? Gs : G;
         }
@@ -517,8 +521,16 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
         for (int f = lane; f < sp.nsf; f += WAVE) S.net.coef[f] = f == fu ? 1.0f : 0.0f;
         wave_sync();
       } else {
-        yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES, true)
-                          : network(md, sp, (global_weights)md.W, S.net, FORCES, true);
+        // the first layer from the lanes' features when it has at most 32 nodes
+        const int first = sp.nnodes[1] <= 32 ? 1 : 0;
+        if (first) {
+          if (P.w_in_lds)
+            first_layer_from_lanes(sp, (lds_weights)lds, lane_f, lane_x, S.net);
+          else
+            first_layer_from_lanes(sp, (global_weights)md.W, lane_f, lane_x, S.net);
+        }
+        yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES, true, first)
+                          : network(md, sp, (global_weights)md.W, S.net, FORCES, true, first);
       }
     }
     const SpeciesDev &sp = *S.spc;

@@ -186,9 +186,37 @@ __device__ inline const SpeciesDev &stage_species(const ModelDev &md, int t, Spe
 typedef const __attribute__((address_space(3))) float *lds_weights;
 typedef const __attribute__((address_space(1))) float *global_weights;
 
+// First weight layer of a species' network (ff_eval,
+// src/ext/feedforward.f90:586-655, layer 0 -> 1) straight from a
+// descriptor spread over the lanes: lane l holds feature f (f < 0: none)
+// with normalized value x.  Each lane forms its products W[f][j] x for the
+// first 32 nodes, and a reduce-scatter over the wavefront sums them, leaving
+// node j in lanes 2j and 2j+1; those lanes add the bias, activate and store
+// the layer's outputs and derivatives in N.act / N.dact (layer 1).  Needs
+// n2 = nnodes[1] <= 32.  network(.., first = 1) then goes on from layer 1.
+template <class WPtr>
+__device__ inline void first_layer_from_lanes(const SpeciesDev &sp, WPtr W, int f, float x, const NetLds &N) {
+  const int lane = lane_id();
+  const int n1 = sp.nnodes[0], n2 = sp.nnodes[1];
+  const WPtr W0 = W + sp.w_off[0];
+  float v[32];
+#pragma unroll
+  for (int j = 0; j < 32; ++j) v[j] = (f >= 0 && j < n2) ? W0[f * n2 + j] * x : 0.0f;
+  const float z = rs_levels<32, 32>(v);
+  const int node = lane >> 1;
+  if ((lane & 1) == 0 && node < n2) {
+    float y, dy;
+    activate(sp.fa[1], z + W0[n1 * n2 + node], y, dy);  // bias column
+    N.act[N.hs + node] = y;
+    N.dact[N.hs + node] = dy;
+  }
+  wave_sync();
+}
+
 // Feed-forward network of one species on the normalized descriptor in
 // N.gp: ff_eval (src/ext/feedforward.f90:586-655), lanes = nodes of a
-// layer.  With grad, also ff_deriv (:661-757) and the chain through the
+// layer.  With first = 1 the first layer's outputs are already in N.act
+// (first_layer_from_lanes) and the forward pass starts from them.  With grad, also ff_deriv (:661-757) and the chain through the
 // normalization and the energy scale: N.coef[f] = dE_i/dG_f.  Returns the
 // network output y (all lanes).  W is lds_weights or global_weights.  With
 // scale_in_coef, N.coef[f] holds the normalization scale of feature f on
@@ -196,11 +224,11 @@ typedef const __attribute__((address_space(1))) float *global_weights;
 // from global memory again.
 template <class WPtr>
 __device__ inline float network(const ModelDev &md, const SpeciesDev &sp, WPtr W, const NetLds &N, bool grad,
-                                bool scale_in_coef = false) {
+                                bool scale_in_coef = false, int first = 0) {
   const int lane = lane_id();
   const int nl = sp.nlayers;
-  const float *in = N.gp;
-  for (int l = 0; l + 1 < nl; ++l) {
+  const float *in = first == 0 ? N.gp : N.act + first * N.hs;
+  for (int l = first; l + 1 < nl; ++l) {
     const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
     const WPtr Wl = W + sp.w_off[l];
     // lanes = (node, part): with NP the node count rounded up to a power of
