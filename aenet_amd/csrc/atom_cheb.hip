@@ -69,6 +69,7 @@ struct ChebLaunch {
   int tune_waves, tune_staged;  // host side: measured launch shape
   int nnorm;     // normalization entries (averages and scales staged after the weights)
   int tile_net;  // networks of a round as one matrix tile (mode 0, one architecture)
+  int ftab;      // entries of the workgroup force table (0: scatter straight to global memory)
 };
 
 // words staged once per workgroup: the weights (rounded to 4 words), then
@@ -281,6 +282,12 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
   constexpr bool tile = MODE == 0 && TILE;
   const int first = blockIdx.x * nw, per_round = gridDim.x * nw;
   const int nrounds = count > first ? (count - first + per_round - 1) / per_round : 0;
+  // Forces of a workgroup that runs one round with its weights staged are
+  // summed per target atom in an LDS table that takes the weights' place
+  // after the round (workgroup-uniform); the wavefront's gradients wait in
+  // its own region until then.
+  const int ftab = MODE == 0 && FORCES && P.w_in_lds && nrounds == 1 ? P.ftab : 0;
+  int agg_g = -1, agg_n = 0;  // this wavefront's atom and neighbor count for the table
   for (int rd = 0; rd < nrounds; ++rd) {
     // the lane index made opaque once per round: lane masks are computed
     // where they are used instead of being hoisted out of the loop and
@@ -730,11 +737,16 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
     wave_sync();
     // F_j -= dE_i/dR_j ; F_i += sum_j dE_i/dR_j
     if constexpr (MODE == 0) {
-      scatter_forces(o.F, b.fmark, g, n, S.j, [&](int q, double (&c)[3]) {
-        c[0] = fs[3 * q];
-        c[1] = fs[3 * q + 1];
-        c[2] = fs[3 * q + 2];
-      });
+      if (ftab) {
+        agg_g = g;
+        agg_n = n;
+      } else {
+        scatter_forces(o.F, b.fmark, g, n, S.j, [&](int q, double (&c)[3]) {
+          c[0] = fs[3 * q];
+          c[1] = fs[3 * q + 1];
+          c[2] = fs[3 * q + 2];
+        });
+      }
     } else {
       double own[3] = {0.0, 0.0, 0.0};
       for (int q = lane; q < n; q += WAVE) {
@@ -755,6 +767,29 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
       }
     }
     wave_sync();
+  }
+  if constexpr (MODE == 0 && FORCES) {
+    if (ftab) {
+      __syncthreads();  // every wavefront is done with the staged weights
+      const ForceTable t = force_table_at(lds, ftab);
+      for (int k = threadIdx.x; k < ftab; k += 64 * nw) t.key[k] = -1;
+      for (int k = threadIdx.x; k < 3 * ftab; k += 64 * nw) t.val[k] = 0ull;
+      __syncthreads();
+      if (agg_g >= 0)
+        scatter_forces_table(t, o.F, b.fmark, agg_g, agg_n, S.j, [&](int q, double (&c)[3]) {
+          c[0] = S.fs[3 * q];
+          c[1] = S.fs[3 * q + 1];
+          c[2] = S.fs[3 * q + 2];
+        });
+      __syncthreads();
+      for (int k = threadIdx.x; k < ftab; k += 64 * nw) {
+        const int a = t.key[k];
+        if (a >= 0) {
+          const long long v[3] = {(long long)t.val[3 * k], (long long)t.val[3 * k + 1], (long long)t.val[3 * k + 2]};
+          force_add_fixed(o.F, a, v);
+        }
+      }
+    }
   }
 }
 
@@ -816,6 +851,11 @@ static ChebLaunch cheb_launch(int mode, const AtomSizes &z) {
   P.nnorm = z.nnorm;
   // the matrix-tile network: structures, one architecture, chosen by measurement
   P.tile_net = mode == 0 && z.net_uniform && z.tune_tile == 1;
+  // the force table in the staged weights' words: the largest power of two
+  // up to 512 entries that fits (7 words each)
+  P.ftab = 0;
+  for (int e = 512; e >= 64 && !P.ftab; e >>= 1)
+    if (force_table_words(e) <= staged_words(P)) P.ftab = e;
   return P;
 }
 

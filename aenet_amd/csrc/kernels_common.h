@@ -200,6 +200,75 @@ __device__ inline void scatter_forces(double *F, int *fmark, int i, int n, const
   }
 }
 
+// Workgroup aggregation of the force scatter.  The atoms of a workgroup
+// round are spatial neighbors and share most of their neighbors, so their
+// contributions are first summed per target atom in an LDS table (open
+// addressing, 2^k entries: key words, then three 64-bit sums per entry)
+// and each target then takes one set of global atomics.  A contribution
+// that finds no free entry goes to global memory directly.  The sums are
+// integers, so the grouping changes no result.
+struct ForceTable {
+  unsigned long long *val;  // [3 * entries]
+  int *key;                 // [entries], -1: free
+  int entries;              // power of two
+};
+
+__host__ __device__ inline int force_table_words(int entries) { return 7 * entries; }
+
+__device__ inline ForceTable force_table_at(float *p, int entries) {
+  ForceTable t;
+  t.val = reinterpret_cast<unsigned long long *>(p);  // 8-byte aligned base
+  t.key = reinterpret_cast<int *>(t.val + 3 * (size_t)entries);
+  t.entries = entries;
+  return t;
+}
+
+__device__ inline void force_table_add(const ForceTable &t, double *F, int atom, const long long (&v)[3]) {
+  unsigned s = ((unsigned)atom * 2654435761u) & (unsigned)(t.entries - 1);
+  for (int probe = 0; probe < t.entries; ++probe) {
+    const int prev = atomicCAS(&t.key[s], -1, atom);
+    if (prev == -1 || prev == atom) {
+#pragma unroll
+      for (int x = 0; x < 3; ++x) atomicAdd(&t.val[3 * s + x], (unsigned long long)v[x]);
+      return;
+    }
+    s = (s + 1) & (unsigned)(t.entries - 1);
+  }
+  force_add_fixed(F, atom, v);  // table full
+}
+
+// scatter_forces into the workgroup's table instead of global memory
+template <class Grad>
+__device__ inline void scatter_forces_table(const ForceTable &t, double *F, int *fmark, int i, int n, const int *J,
+                                            Grad grad) {
+  long long own[3] = {0, 0, 0};
+  bool bad = false;
+  for (int q = lane_id(); q < n; q += WAVE) {
+    double c[3];
+    grad(q, c);
+    const double m[3] = {-c[0], -c[1], -c[2]};
+    long long v[3];
+    const int j = J[q];
+    if (force_fix(m, v)) {
+      force_table_add(t, F, j, v);
+      own[0] -= v[0];
+      own[1] -= v[1];
+      own[2] -= v[2];
+    } else {
+      fmark[j] = 1;
+      bad = true;
+    }
+  }
+  for (int x = 0; x < 3; ++x) own[x] = wave_sum(own[x]);
+  bad = wave_ballot(bad) != 0;
+  if (lane_id() == 0) {
+    if (bad)
+      fmark[i] = 1;
+    else
+      force_table_add(t, F, i, own);
+  }
+}
+
 // inclusive prefix sum over the wavefront
 __device__ __forceinline__ int wave_scan(int x) {
   for (int o = 1; o < WAVE; o <<= 1) {
