@@ -168,6 +168,12 @@ struct BatchDev {
   const double *self_d;       // self-image distances
   int *flags;
   int *fmark;                 // [natoms] a force contribution outside the fixed-point range (see scatter_forces)
+  // per-atom slots for the workgroup force tables' sums (see the atom
+  // kernels' table flush): fcnt[natoms] slots taken, fslot[natoms *
+  // fslots * 3] 64-bit sums; fslots = 0: none (direct atomics)
+  int *fcnt;
+  unsigned long long *fslot;
+  int fslots;
 };
 
 // compact neighbor lists written by the search kernel for the atom kernels

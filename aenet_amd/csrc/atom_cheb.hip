@@ -782,11 +782,21 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
           c[2] = S.fs[3 * q + 2];
         });
       __syncthreads();
+      // each target's sums into one of its slots (one 32-bit atomic for the
+      // slot, plain stores) or, without a free slot, into F directly
       for (int k = threadIdx.x; k < ftab; k += 64 * nw) {
         const int a = t.key[k];
         if (a >= 0) {
           const long long v[3] = {(long long)t.val[3 * k], (long long)t.val[3 * k + 1], (long long)t.val[3 * k + 2]};
-          force_add_fixed(o.F, a, v);
+          const int slot = b.fslots ? atomicAdd(&b.fcnt[a], 1) : b.fslots;
+          if (slot < b.fslots) {
+            unsigned long long *d = b.fslot + 3 * ((size_t)a * b.fslots + slot);
+            d[0] = (unsigned long long)v[0];
+            d[1] = (unsigned long long)v[1];
+            d[2] = (unsigned long long)v[2];
+          } else {
+            force_add_fixed(o.F, a, v);
+          }
         }
       }
     }

@@ -181,6 +181,11 @@ int Batch::allocate(std::string &err) {
   bad |= cell_work_.alloc(4 * (size_t)work_cap_) != 0;
   bad |= scratch_E_.alloc(p.natoms) != 0;
   bad |= fmark_.alloc(p.natoms) != 0;
+  // slots for the atom kernels' workgroup force tables: used only when one
+  // round of wavefronts covers the batch (a few thousand atoms)
+  fslots_ = p.natoms <= FORCE_SLOT_MAX_ATOMS ? FORCE_SLOTS : 0;
+  bad |= fcnt_.alloc(fslots_ ? p.natoms : 1) != 0;
+  bad |= fslot_.alloc(fslots_ ? 3 * (size_t)p.natoms * fslots_ : 1) != 0;
   if (bad) {
     err = "device allocation failed";
     return AENET_STATUS_ERR_MALLOC;
@@ -211,6 +216,7 @@ int Batch::allocate(std::string &err) {
   // must start at zero (each step leaves the counters zeroed again)
   launch_fill_i32(flags_.p, 0, 1, s);
   launch_fill_i32(fmark_.p, 0, p.natoms, s);
+  if (fslots_) launch_fill_i32(fcnt_.p, 0, p.natoms, s);
   launch_fill_i32(cell_count_.p, 0, p.ncells, s);
   launch_fill_i32(cell_fill_.p, 0, p.ncells, s);
   // the (cell, image) combinations depend on the lattice only
@@ -286,6 +292,9 @@ BatchDev Batch::view(const double *d_coords, const int *d_types) const {
   b.self_d = self_d_.p;
   b.flags = flags_.p;
   b.fmark = fmark_.p;
+  b.fcnt = fcnt_.p;
+  b.fslot = fslot_.p;
+  b.fslots = fslots_;
   return b;
 }
 

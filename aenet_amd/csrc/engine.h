@@ -179,7 +179,9 @@ class Batch {
   DBuf<double> self_d_;
   DBuf<double> s_;
   DBuf<int> cellof_, cell_count_, cell_fill_, cell_start_, cell_atoms_, cell_tmp_, combo_count_, combos_, flags_;
-  DBuf<int> cell_work_, fmark_;
+  DBuf<int> cell_work_, fmark_, fcnt_;
+  DBuf<unsigned long long> fslot_;
+  int fslots_ = 0;
   DBuf<double> cell_ctr_, sorted_s_;
   int combo_cap_ = 0;
   int kept_cap_ = AENET_AMD_KEPT_CAP;

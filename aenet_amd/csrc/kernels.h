@@ -14,6 +14,13 @@ namespace aenet_amd {
 constexpr int CHEB_MAX_WAVES = 16;
 constexpr int ATOM_MAX_WAVES = 4;
 
+// per-atom slots for the workgroup force tables' sums: a batch of up to
+// FORCE_SLOT_MAX_ATOMS atoms (one round of wavefronts covers it) gets
+// FORCE_SLOTS per atom (the workgroups that share a target); a sum that
+// finds them taken goes to the atom's fixed-point F directly
+constexpr int FORCE_SLOTS = 32;
+constexpr int FORCE_SLOT_MAX_ATOMS = 16384;
+
 // kernels.hip
 void launch_fill_i32(int *p, int v, long n, hipStream_t s);
 void launch_fill_f64(double *p, double v, long n, hipStream_t s);

@@ -412,16 +412,50 @@ __global__ __launch_bounds__(256) void k_struct_energy(BatchDev b, ModelDev md, 
   __shared__ double st[256], sc[256];
   __shared__ double e_free[AENET_AMD_MAX_TYPES];
   if ((int)blockIdx.x >= nsum) {
-    const int g = (blockIdx.x - nsum) * 256 + threadIdx.x;
-    if (g < b.natoms) {
-      long long *Fi = reinterpret_cast<long long *>(F) + 3 * (size_t)g;
-      const long long v0 = Fi[0], v1 = Fi[1], v2 = Fi[2];
-      const bool bad = b.fmark[g] != 0;
-      const double nan = __builtin_nan("");
-      F[3 * (size_t)g] = bad ? nan : (double)v0 * fscale;
-      F[3 * (size_t)g + 1] = bad ? nan : (double)v1 * fscale;
-      F[3 * (size_t)g + 2] = bad ? nan : (double)v2 * fscale;
-      if (bad) b.fmark[g] = 0;
+    const int cb = blockIdx.x - nsum;
+    const double nan = __builtin_nan("");
+    if (b.fslots) {
+      // 32 lanes per atom, lane k adds slot k: one memory round trip, then
+      // an integer butterfly over the half-wavefront (the order of integer
+      // sums changes nothing)
+      static_assert(FORCE_SLOTS == 32, "one lane per slot");
+      const int g = cb * 8 + (int)(threadIdx.x >> 5), k = threadIdx.x & 31;
+      long long u[3] = {0, 0, 0};
+      if (g < b.natoms) {
+        if (k < min(b.fcnt[g], b.fslots)) {
+          const unsigned long long *sl = b.fslot + 3 * ((size_t)g * b.fslots + k);
+          u[0] = (long long)sl[0];
+          u[1] = (long long)sl[1];
+          u[2] = (long long)sl[2];
+        }
+        if (k == 0) {  // the sums that went to F directly
+          const long long *Fi = reinterpret_cast<const long long *>(F) + 3 * (size_t)g;
+          u[0] += Fi[0];
+          u[1] += Fi[1];
+          u[2] += Fi[2];
+        }
+      }
+#pragma unroll
+      for (int x = 0; x < 3; ++x) u[x] = butterfly_sum<16>(u[x]);
+      if (g < b.natoms && k == 0) {
+        const bool bad = b.fmark[g] != 0;
+        F[3 * (size_t)g] = bad ? nan : (double)u[0] * fscale;
+        F[3 * (size_t)g + 1] = bad ? nan : (double)u[1] * fscale;
+        F[3 * (size_t)g + 2] = bad ? nan : (double)u[2] * fscale;
+        if (bad) b.fmark[g] = 0;
+        b.fcnt[g] = 0;
+      }
+    } else {
+      const int g = cb * 256 + threadIdx.x;
+      if (g < b.natoms) {
+        const long long *Fi = reinterpret_cast<const long long *>(F) + 3 * (size_t)g;
+        const long long v0 = Fi[0], v1 = Fi[1], v2 = Fi[2];
+        const bool bad = b.fmark[g] != 0;
+        F[3 * (size_t)g] = bad ? nan : (double)v0 * fscale;
+        F[3 * (size_t)g + 1] = bad ? nan : (double)v1 * fscale;
+        F[3 * (size_t)g + 2] = bad ? nan : (double)v2 * fscale;
+        if (bad) b.fmark[g] = 0;
+      }
     }
     return;
   }
@@ -896,7 +930,7 @@ void launch_neighbor_lists(const BatchDev &b, const OutDev &o, hipStream_t s) {
 void launch_struct_energy(const BatchDev &b, const ModelDev &md, const double *E_atom, double *E_tot,
                           double *E_coh, double *F, double fscale, hipStream_t s) {
   const int nsum = E_tot ? b.nstruct : 0;
-  const int nconv = F ? nblk(b.natoms, 256) : 0;
+  const int nconv = F ? (b.fslots ? nblk(b.natoms, 8) : nblk(b.natoms, 256)) : 0;
   if (nsum + nconv > 0)
     hipLaunchKernelGGL(k_struct_energy, dim3(nsum + nconv), dim3(256), 0, s, b, md, E_atom, E_tot, E_coh, nsum, F,
                        fscale * FORCE_FIX_UNIT);
