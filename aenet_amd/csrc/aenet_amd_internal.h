@@ -56,12 +56,28 @@ struct BPDev {
   float ang_rcmax[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];  // largest cutoff of the block
   int ang_g5[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];       // block holds G5 (no r_jk cutoff)
   int ang_one_seg[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];  // all functions: one cutoff, one kind
+  // one segment and every lambda +-1: the block has BPAngK constants (at
+  // ang_p0, sign-sorted) with ang_np of lambda = +1 first
+  int ang_fast[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];
+  int ang_np[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];
 };
 
 // one radial (G1/G2/G3) or angular (G4/G5) function record
 struct BPFunc {
   int kind;      // 1..5
   float Rc, p1, p2, p3;   // G2: Rs, eta; G3: kappa; G4/G5: lambda, zeta, eta
+};
+
+// Constants of one G4/G5 function of a block the angular fast path takes,
+// with arg = (1 + lambda cos) / 2:
+//   G = 2 arg^zeta exp(-eta S) C = 2 arg 2^(A log2(arg) + B S) C
+struct BPAngK {
+  float A;    // zeta - 1
+  float B;    // -eta log2(e)
+  float K1;   // zeta lambda / 2: d/dcos of the function is K1 2^(..) C times 2
+  float K2;   // -2 eta: d/d(r^2) of the exponent's eta S term
+  int feat;   // the function's place in the block (feature ang_off + feat)
+  int pad[3];
 };
 
 // Per-structure plan: lattice-derived data computed on the host with the
@@ -129,6 +145,7 @@ struct ModelDev {
   const double *norm_scale;
   const BPDev *bp;
   const BPFunc *bpf;
+  const BPAngK *bpk;  // parallel to bpf (angular fast-path blocks only)
   int ntypes;
   int desc;
 };
@@ -174,6 +191,11 @@ struct BatchDev {
   int *fcnt;
   unsigned long long *fslot;
   int fslots;
+  // the BP kernel's per-wavefront triplet lists: trip[slot * trip_cap ..]
+  // for wavefront slots below trip_slots (null: none)
+  int *trip;
+  int trip_cap;
+  int trip_slots;
 };
 
 // compact neighbor lists written by the search kernel for the atom kernels

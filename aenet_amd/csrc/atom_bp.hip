@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "aenet_amd_internal.h"
 #include "atom_common.h"
@@ -54,6 +55,7 @@ constexpr int MT = AENET_AMD_MAX_TYPES;
 // per-function branches (cutoff, kind, lambda) wave-uniform.
 typedef const __attribute__((address_space(4))) BPFunc *bp_funcs;
 typedef const __attribute__((address_space(4))) BPDev *bp_tables;
+typedef const __attribute__((address_space(4))) BPAngK *bp_angk;
 
 // 2^x and log2(x) as the hardware instructions (1 ulp).  The library forms
 // add scaling for subnormal inputs and results only: the log2 arguments here
@@ -182,8 +184,10 @@ __device__ __forceinline__ void for_triplets(const BPWave &S, int b2, int m2, in
   const int ncand = same ? m2 * (m2 - 1) / 2 : m2 * m3;
   const float inv_m2 = m2 > 0 ? 1.0f / (float)m2 : 0.0f;
   const float inv_m3 = m3 > 0 ? 1.0f / (float)m3 : 0.0f;
+  // one call site of op (it is inlined once): groups of 64 while
+  // candidates remain, then the rest
   int qn = 0;
-  for (int base = 0; base < ncand; base += WAVE) {
+  for (int base = 0;;) {
     const int p = base + lane;
     bool ok = false;
     int packed = 0;
@@ -228,23 +232,56 @@ __device__ __forceinline__ void for_triplets(const BPWave &S, int b2, int m2, in
       }
       packed = ja | (kb << 16);
     }
-    const unsigned long long m = wave_ballot(ok);
-    if (ok) S.queue[qn + mask_prefix(m)] = packed;
-    qn += popc64(m);
-    wave_sync();
-    if (qn >= WAVE) {
-      const int e = S.queue[lane];
-      op(e & 0xffff, e >> 16, true);
-      wave_sync();
-      if (lane < qn - WAVE) S.queue[lane] = S.queue[WAVE + lane];
-      qn -= WAVE;
+    if (base < ncand) {
+      const unsigned long long m = wave_ballot(ok);
+      if (ok) S.queue[qn + mask_prefix(m)] = packed;
+      qn += popc64(m);
+      base += WAVE;
       wave_sync();
     }
+    const bool more = base < ncand;
+    if (qn >= WAVE || (!more && qn > 0)) {
+      const int e = lane < qn ? S.queue[lane] : 0;
+      op(e & 0xffff, e >> 16, lane < qn);
+      wave_sync();
+      if (lane < qn - WAVE) S.queue[lane] = S.queue[WAVE + lane];
+      qn = max(qn - WAVE, 0);
+      wave_sync();
+    }
+    if (!more && qn == 0) break;
   }
-  if (qn > 0) {
-    const int e = lane < qn ? S.queue[lane] : 0;
-    op(e & 0xffff, e >> 16, lane < qn);
-    wave_sync();
+}
+
+// Wraps an op of for_triplets and writes the triplets it is handed, in
+// that order, to list[0 ..]: entries beyond cap are counted, not written.
+// for_triplets hands an op a prefix of the lanes as valid, with every lane
+// active.
+template <class Op>
+struct RecordTriplets {
+  Op &op;
+  int *list;
+  int cap;
+  int n;  // triplets handed over so far
+  __device__ __forceinline__ void operator()(int ja, int kb, bool valid) {
+    const int i = n + lane_id();
+    if (valid && i < cap) list[i] = ja | (kb << 16);
+    n += popc64(wave_ballot(valid));
+    op(ja, kb, valid);
+  }
+};
+
+// the n triplets of a list written by RecordTriplets through op, in the
+// groups of 64 that for_triplets formed; the next group's load is issued
+// before the current group's op
+template <class Op>
+__device__ __forceinline__ void replay_triplets(const int *list, int n, Op &op) {
+  const int lane = lane_id();
+  int e = lane < n ? list[lane] : 0;
+  for (int base = 0; base < n; base += WAVE) {
+    const int i = base + lane;
+    const int next = i + WAVE < n ? list[i + WAVE] : 0;
+    op(e & 0xffff, e >> 16, i < n);
+    e = next;
   }
 }
 
@@ -407,11 +444,118 @@ struct AngForceOp {
   }
 };
 
+// Fast path of an angular block with one cutoff, one kind and lambda = +-1
+// (BPAngK constants, sign-sorted: the first np functions have lambda = +1).
+// Per triplet log2 of both arguments once; per function
+//   P = 2^(A L + B S)      (G = 2 arg P C)
+// and no per-function branches on the parameters.
+struct AngFastGeom {
+  float Lp, Lm, argp, argm, S;
+  TripCut c;
+  float C;  // cutoff product of the segment's kind
+};
+
+__device__ __forceinline__ void fast_geom(const Trip &T, bool g4, float rc, const CutTab &tab, int ja, int kb,
+                                          AngFastGeom &q) {
+  q.argp = 0.5f * (1.0f + T.cosv);
+  q.argm = 0.5f * (1.0f - T.cosv);
+  q.Lp = log2_hw(fmaxf(q.argp, 1.0e-30f));
+  q.Lm = log2_hw(fmaxf(q.argm, 1.0e-30f));
+  const float S2 = T.rj * T.rj + T.rk * T.rk;
+  q.S = g4 ? S2 + T.rjk * T.rjk : S2;
+  trip_cut(T, rc, q.c, tab, ja, kb);
+  q.C = g4 ? q.c.cj * q.c.ck * q.c.cjk : q.c.cj * q.c.ck;
+}
+
+// energy pass, fast path: acc[k] += P_k (2 arg C) for function k of the
+// chunk (sign-sorted order)
+struct AngValueFastOp {
+  const float *R, *r;
+  CutTab tab;
+  bp_angk K;  // the chunk's constants
+  int nf, np;  // functions in this chunk, of which lambda = +1 first
+  float rc;
+  bool g4;
+  float acc[ACH];
+  __device__ __forceinline__ void operator()(int ja, int kb, bool valid) {
+    if (!valid) return;
+    const Trip T = make_trip(R, r, ja, kb);
+    AngFastGeom q;
+    fast_geom(T, g4, rc, tab, ja, kb, q);
+    const float aCp = 2.0f * q.argp * q.C, aCm = 2.0f * q.argm * q.C;
+#pragma unroll
+    for (int k = 0; k < ACH; ++k) {
+      if (k < nf) {
+        const bool pos = k < np;
+        const float x = fmaf(K[k].A, pos ? q.Lp : q.Lm, K[k].B * q.S);
+        acc[k] = fmaf(exp2_hw(x), pos ? aCp : aCm, acc[k]);
+      }
+    }
+  }
+};
+
+// force pass, fast path: the sums of the generic force op from the
+// constants, the block's lambda = +1 functions then its lambda = -1 ones
+struct AngForceFastOp {
+  const float *R, *r;
+  CutTab tab;
+  float *fs;
+  bp_angk K;
+  const float *coef;  // dE/dG of the block's functions (file order)
+  int nf, np;
+  float rc;
+  bool g4;
+  __device__ __forceinline__ void operator()(int ja, int kb, bool valid) {
+    if (!valid) return;
+    const Trip T = make_trip(R, r, ja, kb);
+    AngFastGeom q;
+    fast_geom(T, g4, rc, tab, ja, kb, q);
+    // s_cos = sum w K1 P, s_E = sum w P arg, s_eta = sum w K2 P arg (P
+    // without its factor 2, applied below)
+    float sc = 0.0f, sep = 0.0f, sem = 0.0f, shp = 0.0f, shm = 0.0f;
+#pragma unroll 4
+    for (int k = 0; k < np; ++k) {
+      const float t = coef[K[k].feat] * exp2_hw(fmaf(K[k].A, q.Lp, K[k].B * q.S));
+      sc = fmaf(K[k].K1, t, sc);
+      sep += t;
+      shp = fmaf(K[k].K2, t, shp);
+    }
+#pragma unroll 4
+    for (int k = np; k < nf; ++k) {
+      const float t = coef[K[k].feat] * exp2_hw(fmaf(K[k].A, q.Lm, K[k].B * q.S));
+      sc = fmaf(K[k].K1, t, sc);
+      sem += t;
+      shm = fmaf(K[k].K2, t, shm);
+    }
+    const float s_cos = 2.0f * sc;
+    const float s_E = 2.0f * (q.argp * sep + q.argm * sem);
+    const float s_eta = 2.0f * (q.argp * shp + q.argm * shm);
+    const TripCut &c = q.c;
+    const float cjk = g4 ? c.cjk : 1.0f;
+    const float C = q.C;
+    const float dcos = s_cos * C;
+    const float dRj = s_E * (c.dcj * c.ck * cjk) + s_eta * T.rj * C;
+    const float dRk = s_E * (c.cj * c.dck * cjk) + s_eta * T.rk * C;
+    const float dRjk = g4 ? s_E * (c.cj * c.ck * c.dcjk) + s_eta * T.rjk * C : 0.0f;
+    const float inv_rjk = T.rjk > 1.0e-8f ? __builtin_amdgcn_rcpf(T.rjk) : 1.0f;
+    const float cj = dcos * T.ij, ck = dcos * T.ik;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      const float e = dRjk * T.d[x] * inv_rjk;
+      const float gj = dRj * T.uj[x] + cj * (T.uk[x] - T.cosv * T.uj[x]) - e;
+      const float gk = dRk * T.uk[x] + ck * (T.uj[x] - T.cosv * T.uk[x]) + e;
+      atomicAdd(&fs[3 * ja + x], gj);
+      atomicAdd(&fs[3 * kb + x], gk);
+    }
+  }
+};
+
 // MODE 0: structures (compact lists, fixed-point atomic force scatter)
 // MODE 1: explicit environments (per-neighbor forces out)
+// Registers are held to what BP_SIMD_WAVES wavefronts per SIMD allow.
 template <int MODE, bool FORCES>
-__global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, BatchDev b, NbDev nbl, OutDev o,
-                                                                EnvDev env, BPLaunch P) {
+__global__ __launch_bounds__(64 * ATOM_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(BP_SIMD_WAVES))) void k_atom_bp(
+    ModelDev md, BatchDev b, NbDev nbl, OutDev o, EnvDev env, BPLaunch P) {
   extern __shared__ float lds[];
   const int lane = lane_id();
   const int wave = threadIdx.x / WAVE;
@@ -425,6 +569,11 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
   const bp_tables bpt = (bp_tables)md.bp;
   const bp_funcs bpf = (bp_funcs)md.bpf;
   const int nt = bpt->ntypes;
+  // this wavefront's triplet list (forces of structures only): per angular
+  // block a count word (-1: the block's triplets are not listed), then its
+  // triplets; blocks are listed while the count word fits
+  const int tslot = blockIdx.x * nw + wave;
+  int *const tlist = MODE == 0 && FORCES && b.trip && tslot < b.trip_slots ? b.trip + (size_t)tslot * b.trip_cap : nullptr;
 
   const int count = P.active ? *P.nactive : P.natoms;
   for (int k = blockIdx.x * nw + wave; k < count; k += gridDim.x * nw) {
@@ -562,7 +711,10 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
         }
       }
     }
-    // angular blocks, unordered type pairs t2 <= t3
+    // angular blocks, unordered type pairs t2 <= t3; with a triplet list,
+    // the first chunk of a block records the triplets it is handed and the
+    // block's later chunks (and the force pass) replay them
+    int toff = 0;  // words of the list in use
     for (int t2 = 0; t2 < nt; ++t2) {
       for (int t3 = t2; t3 < nt; ++t3) {
         const int key3 = (row * MT + t2) * MT + t3;
@@ -572,24 +724,63 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
         const int b2 = S.bstart[t2], m2 = S.bstart[t2 + 1] - b2;
         const int b3 = S.bstart[t3], m3 = S.bstart[t3 + 1] - b3;
         const float rc = bpt->ang_rcmax[key3];
+        int tn = -1;  // triplets of this block in the list
+        const bool listed = tlist && toff < b.trip_cap;
+        int *const tl = listed ? tlist + toff + 1 : nullptr;
+        const int tcap = listed ? b.trip_cap - toff - 1 : 0;
+        // the chunk's triplets through op: from the list, or found (and
+        // recorded by the first chunk while there is room)
+        auto traverse = [&](auto &op, int c0) {
+          if (tn >= 0) {
+            replay_triplets(tl, tn, op);
+          } else {
+            RecordTriplets<std::remove_reference_t<decltype(op)>> rec{op, tl, c0 == 0 ? tcap : 0, 0};
+            for_triplets(S, b2, m2, b3, m3, t2 == t3, bpt->ang_g5[key3] != 0, rc * rc, rec);
+            if (listed && c0 == 0 && rec.n <= tcap) tn = rec.n;
+          }
+        };
+        const bool fast = bpt->ang_fast[key3] != 0;
+        const bp_angk K = (bp_angk)md.bpk + bpt->ang_p0[key3];
         for (int c0 = 0; c0 < nf; c0 += ACH) {
-          AngValueOp op;
-          op.R = S.R;
-          op.r = S.r;
-          op.tab = tab;
-          op.fn = fn + c0;
-          op.nf = min(ACH, nf - c0);
+          double G;
+          if (fast) {
+            AngValueFastOp op;
+            op.R = S.R;
+            op.r = S.r;
+            op.tab = tab;
+            op.K = K + c0;
+            op.nf = min(ACH, nf - c0);
+            op.np = min(op.nf, max(0, bpt->ang_np[key3] - c0));
+            op.rc = fn[0].Rc;
+            op.g4 = fn[0].kind == 4;
 #pragma unroll
-          for (int k = 0; k < ACH; ++k) op.acc[k] = 0.0f;
-          for_triplets(S, b2, m2, b3, m3, t2 == t3, bpt->ang_g5[key3] != 0, rc * rc, op);
-          const double G = reduce_scatter<ACH>(op.acc);
+            for (int k = 0; k < ACH; ++k) op.acc[k] = 0.0f;
+            traverse(op, c0);
+            G = reduce_scatter<ACH>(op.acc);
+          } else {
+            AngValueOp op;
+            op.R = S.R;
+            op.r = S.r;
+            op.tab = tab;
+            op.fn = fn + c0;
+            op.nf = min(ACH, nf - c0);
+#pragma unroll
+            for (int k = 0; k < ACH; ++k) op.acc[k] = 0.0f;
+            traverse(op, c0);
+            G = reduce_scatter<ACH>(op.acc);
+          }
           const int idx = lane >> 1;
           if ((lane & 1) == 0 && c0 + idx < nf) {
-            const int ft = off + c0 + idx;
+            // (the fast path's functions are sign-sorted)
+            const int ft = off + (fast ? K[c0 + idx].feat : c0 + idx);
             const double Gs = md.norm_scale[sp.norm_off + ft] * (G - md.norm_avg[sp.norm_off + ft]);
             S.net.gp[ft] = (float)Gs;
             if (o.G) o.G[(size_t)g * o.G_stride + ft] = o.G_scaled ? Gs : G;
           }
+        }
+        if (listed) {
+          if (lane == 0) tlist[toff] = tn;
+          toff += 1 + max(tn, 0);
         }
       }
     }
@@ -652,6 +843,10 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
       }
     }
     wave_sync();
+    // the value pass's list entries were written by other lanes of this
+    // wavefront
+    if (tlist) __threadfence_block();
+    toff = 0;
     for (int t2 = 0; t2 < nt; ++t2) {
       for (int t3 = t2; t3 < nt; ++t3) {
         const int key3 = (row * MT + t2) * MT + t3;
@@ -660,16 +855,45 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_atom_bp(ModelDev md, Ba
         const int b2 = S.bstart[t2], m2 = S.bstart[t2 + 1] - b2;
         const int b3 = S.bstart[t3], m3 = S.bstart[t3 + 1] - b3;
         const float rc = bpt->ang_rcmax[key3];
-        AngForceOp op;
-        op.R = S.R;
-        op.r = S.r;
-        op.tab = tab;
-        op.fs = S.fs;
-        op.fn = bpf + bpt->ang_p0[key3];
-        op.coef = coef + off;
-        op.nf = nf;
-        op.one_seg = bpt->ang_one_seg[key3] != 0;
-        for_triplets(S, b2, m2, b3, m3, t2 == t3, bpt->ang_g5[key3] != 0, rc * rc, op);
+        int tn = -1;
+        if (tlist && toff < b.trip_cap) {
+          tn = uniform(tlist[toff]);
+          toff += 1;
+        }
+        const int *const tl = tn >= 0 ? tlist + toff : nullptr;
+        if (tn > 0) toff += tn;
+        auto traverse = [&](auto &op) {
+          if (tn >= 0)
+            replay_triplets(tl, tn, op);
+          else
+            for_triplets(S, b2, m2, b3, m3, t2 == t3, bpt->ang_g5[key3] != 0, rc * rc, op);
+        };
+        const bp_funcs fn = bpf + bpt->ang_p0[key3];
+        if (bpt->ang_fast[key3]) {
+          AngForceFastOp op;
+          op.R = S.R;
+          op.r = S.r;
+          op.tab = tab;
+          op.fs = S.fs;
+          op.K = (bp_angk)md.bpk + bpt->ang_p0[key3];
+          op.coef = coef + off;
+          op.nf = nf;
+          op.np = bpt->ang_np[key3];
+          op.rc = fn[0].Rc;
+          op.g4 = fn[0].kind == 4;
+          traverse(op);
+        } else {
+          AngForceOp op;
+          op.R = S.R;
+          op.r = S.r;
+          op.tab = tab;
+          op.fs = S.fs;
+          op.fn = fn;
+          op.coef = coef + off;
+          op.nf = nf;
+          op.one_seg = bpt->ang_one_seg[key3] != 0;
+          traverse(op);
+        }
       }
     }
     wave_sync();

@@ -42,6 +42,7 @@ int DeviceModel::upload(const PotSet &ps, hipStream_t s) {
   if (scale.upload(ps.norm_scale.data(), ps.norm_scale.size(), s)) return AENET_STATUS_ERR_DEVICE;
   if (bp.upload(&ps.bp, 1, s)) return AENET_STATUS_ERR_DEVICE;
   if (bpf.upload(ps.bpf.data(), ps.bpf.size(), s)) return AENET_STATUS_ERR_DEVICE;
+  if (bpk.upload(ps.bpk.data(), ps.bpk.size(), s)) return AENET_STATUS_ERR_DEVICE;
   if (hipStreamSynchronize(s) != hipSuccess) return AENET_STATUS_ERR_DEVICE;
   return AENET_STATUS_OK;
 }
@@ -54,6 +55,7 @@ ModelDev DeviceModel::view() const {
   m.norm_scale = scale.p;
   m.bp = bp.p;
   m.bpf = bpf.p;
+  m.bpk = bpk.p;
   m.ntypes = ntypes;
   m.desc = desc;
   return m;
@@ -101,6 +103,7 @@ static double occupied_fraction(const double *L, int n, const double *cart, doub
 int Batch::create(const PotSet &ps, int nstruct, const int *natoms, const double *latt, const int *pbc,
                   const double *coords, std::string &err) {
   rc_max_ = ps.Rc_max;
+  bp_ = ps.desc != DESC_CHEBYSHEV;
   int st = build_plan(ps.Rc_max, nstruct, natoms, latt, pbc, coords, plan_, err);
   if (st) return st;
   occ_.assign(nstruct, 1.0);
@@ -145,6 +148,7 @@ int Batch::replan(const double *coords, std::string &err) {
 
 int Batch::create_lattice_coords(const PotSet &ps, int natoms, const double *latt, int pbc, std::string &err) {
   rc_max_ = ps.Rc_max;
+  bp_ = ps.desc != DESC_CHEBYSHEV;
   lattice_coords_ = true;
   int st = build_plan_lattice(ps.Rc_max, natoms, latt, pbc, plan_, err);
   if (st) return st;
@@ -186,6 +190,14 @@ int Batch::allocate(std::string &err) {
   fslots_ = p.natoms <= FORCE_SLOT_MAX_ATOMS ? FORCE_SLOTS : 0;
   bad |= fcnt_.alloc(fslots_ ? p.natoms : 1) != 0;
   bad |= fslot_.alloc(fslots_ ? 3 * (size_t)p.natoms * fslots_ : 1) != 0;
+  if (bp_ && !current_arena()) {
+    // one list per wavefront the BP kernel can hold resident at once (a
+    // persistent grid has at most a wavefront per atom, rounded up to a
+    // workgroup); none in a fixed scratch arena, where the force pass takes
+    // the candidate pass again
+    trip_slots_ = std::min(p.natoms + 64, device_limits().cus * BP_TRIP_WAVES_PER_CU);
+    bad |= trip_.alloc((size_t)trip_slots_ * BP_TRIP_CAP) != 0;
+  }
   if (bad) {
     err = "device allocation failed";
     return AENET_STATUS_ERR_MALLOC;
@@ -295,6 +307,9 @@ BatchDev Batch::view(const double *d_coords, const int *d_types) const {
   b.fcnt = fcnt_.p;
   b.fslot = fslot_.p;
   b.fslots = fslots_;
+  b.trip = bp_ ? trip_.p : nullptr;
+  b.trip_cap = BP_TRIP_CAP;
+  b.trip_slots = bp_ ? trip_slots_ : 0;
   return b;
 }
 

@@ -88,6 +88,7 @@ struct DeviceModel {
   DBuf<double> avg, scale;
   DBuf<BPDev> bp;
   DBuf<BPFunc> bpf;
+  DBuf<BPAngK> bpk;
   int ntypes = 0, desc = 0, max_rN = 0, max_aN = 0, nsf_max = 0, wsize = 0, nl_max = 0, hmax = 1, nnorm = 0, net_uniform = 0;
   double rc_max = 0.0;
   AtomSizes sizes(int natoms, int cap, const int *active = nullptr, const int *nactive = nullptr) const {
@@ -182,6 +183,10 @@ class Batch {
   DBuf<int> cell_work_, fmark_, fcnt_;
   DBuf<unsigned long long> fslot_;
   int fslots_ = 0;
+  // triplet lists of the BP kernel (batches made for BP potentials)
+  bool bp_ = false;
+  DBuf<int> trip_;
+  int trip_slots_ = 0;
   DBuf<double> cell_ctr_, sorted_s_;
   int combo_cap_ = 0;
   int kept_cap_ = AENET_AMD_KEPT_CAP;

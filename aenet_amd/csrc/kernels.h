@@ -21,6 +21,14 @@ constexpr int ATOM_MAX_WAVES = 4;
 constexpr int FORCE_SLOTS = 32;
 constexpr int FORCE_SLOT_MAX_ATOMS = 16384;
 
+// the BP kernel's triplet lists: the angular value pass records the
+// triplets that contribute, the force pass replays them.  One list of
+// BP_TRIP_CAP entries per resident wavefront (a block that does not fit
+// takes the candidate pass again); BP_TRIP_WAVES_PER_CU bounds the
+// resident wavefronts of a CU
+constexpr int BP_TRIP_CAP = 4096;
+constexpr int BP_TRIP_WAVES_PER_CU = 16;
+
 // kernels.hip
 void launch_fill_i32(int *p, int v, long n, hipStream_t s);
 void launch_fill_f64(double *p, double v, long n, hipStream_t s);

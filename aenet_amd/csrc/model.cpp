@@ -301,6 +301,7 @@ int PotSet::finalize(std::string &err) {
     std::memset(&bp, 0, sizeof(bp));
     bp.ntypes = ntypes;
     bpf.clear();
+    bpk.clear();
     // registration lists per (row, funct, t2[, t3])
     for (int row = 0; row < ntypes; ++row) {
       const SetupHost &s = pots[row].stp;
@@ -342,6 +343,35 @@ int PotSet::finalize(std::string &err) {
             const BPFunc &f0 = bpf[bp.ang_p0[key3]];
             if (bpf[k].Rc != f0.Rc || bpf[k].kind != f0.kind) bp.ang_one_seg[key3] = 0;
           }
+          // the fast path's constants: lambda = +1 first, then -1, each in
+          // file order
+          bp.ang_fast[key3] = 0;
+          bp.ang_np[key3] = 0;
+          if (bp.ang_one_seg[key3] && bp.ang_n[key3] > 0) {
+            const int p0 = bp.ang_p0[key3], n = bp.ang_n[key3];
+            bool lam_pm1 = true;
+            for (int k = p0; k < p0 + n; ++k) lam_pm1 = lam_pm1 && (bpf[k].p1 == 1.0f || bpf[k].p1 == -1.0f);
+            if (lam_pm1) {
+              bpk.resize(bpf.size());
+              int q = p0;
+              for (int sign = 1; sign >= -1; sign -= 2)
+                for (int k = p0; k < p0 + n; ++k) {
+                  const BPFunc &f = bpf[k];
+                  if (f.p1 != (float)sign) continue;
+                  BPAngK c{};
+                  c.A = f.p2 - 1.0f;
+                  c.B = (float)(-(double)f.p3 * 1.4426950408889634);
+                  c.K1 = 0.5f * f.p2 * f.p1;
+                  c.K2 = -2.0f * f.p3;
+                  c.feat = k - p0;
+                  bpk[q++] = c;
+                  if (sign == 1) ++bp.ang_np[key3];
+                }
+              bp.ang_fast[key3] = 1;
+            }
+          }
+          bp.ang_fast[key3b] = bp.ang_fast[key3];
+          bp.ang_np[key3b] = bp.ang_np[key3];
           bp.ang_off[key3b] = bp.ang_off[key3];
           bp.ang_p0[key3b] = bp.ang_p0[key3];
           bp.ang_n[key3b] = bp.ang_n[key3];
@@ -358,6 +388,7 @@ int PotSet::finalize(std::string &err) {
           return AENET_STATUS_ERR_IO;
         }
     }
+    bpk.resize(bpf.size());
     for (int t = 0; t < ntypes; ++t) {
       if (species[t].bp_row < 0) {
         err = "central species missing from its own ENV list";

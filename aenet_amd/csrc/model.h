@@ -56,6 +56,7 @@ struct PotSet {
   std::vector<double> norm_avg, norm_scale;
   BPDev bp{};
   std::vector<BPFunc> bpf;
+  std::vector<BPAngK> bpk;  // parallel to bpf
   int max_rN = 0, max_aN = 0;
 
   // called when the last potential is loaded (aenet_load_potential,

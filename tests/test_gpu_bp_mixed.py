@@ -48,3 +48,67 @@ def test_bp_mixed_kinds_and_cutoffs(name, device):
     assert np.abs(g["forces"]).max() > 0.1  # the forces are not trivially zero
     assert dE < E_TOL
     assert dF < F_TOL
+
+
+def _bp3_variant(tmp_path, kind):
+    """bp3 potentials changed so that the kernel's angular fast path (blocks
+    of one cutoff, one kind and lambda = +-1, functions sign-sorted) meets
+    the cases bp3 itself does not hold: 'g5' turns every G4 into G5 (no r_jk
+    cutoff, S = r_ij^2 + r_ik^2); 'signs' gives the (A, A) block lambda = -1
+    only and the (B, B) block lambda = +1 only."""
+    from aenet_amd.nnfile import read_nn, write_nn
+    files = {}
+    for t in "ABC":
+        p = read_nn(pot_files("bp3", "ABC")[t])
+        sf = np.array(p["sf"])
+        par = np.array(p["sfparam"], dtype=np.float64).reshape(int(p["nsf"]), -1)
+        env = np.array(p["sfenv"]).reshape(int(p["nsf"]), 2)
+        if kind == "g5":
+            sf[sf == 4] = 5
+        else:
+            for i in range(len(sf)):
+                if sf[i] == 4 and tuple(env[i]) == (1, 1):
+                    par[i, 1] = -1.0
+                elif sf[i] == 4 and tuple(env[i]) == (2, 2):
+                    par[i, 1] = 1.0
+        p["sf"] = sf
+        p["sfparam"] = par
+        path = str(tmp_path / ("%s.%s.nn" % (kind, t)))
+        write_nn(path, p)
+        files[t] = path
+    return files
+
+
+@pytest.mark.parametrize("kind", ["g5", "signs"])
+def test_bp_angular_fast_path_variants(kind, tmp_path, device):
+    """Energies and forces of the fast-path variants against the FP64
+    oracle: a displaced 108-atom FCC cell of three species and a 40-atom
+    cluster.  Bars as above."""
+    from aenet_amd import synth
+    from aenet_amd.core import ANNPotentials
+    from oracle_lib import Oracle
+    types = ["A", "B", "C"]
+    files = _bp3_variant(tmp_path, kind)
+    rng = np.random.Generator(np.random.PCG64(7))
+    cell, pos = synth.fcc(3.6, (3, 3, 3))
+    cases = [(cell, True, synth.displaced(pos, 0.1, 8))]
+    big_cell, big = synth.fcc(3.6, (4, 4, 4))
+    big = big - big.mean(axis=0)
+    order = np.argsort(np.sum(big * big, axis=1), kind="stable")
+    cases.append((np.eye(3) * 30.0, False, synth.displaced(big[order[:40]], 0.1, 9)))
+    orc = Oracle(types, files)
+    ann = ANNPotentials(files)
+    try:
+        for cell_, pbc, p in cases:
+            at = rng.integers(1, 4, len(p)).astype(np.int32)
+            print("case", kind, len(p), "atoms pbc", pbc, flush=True)
+            ref = orc.structure(cell_, pbc, at, p)
+            r = ann.structure_energy_and_forces(cell_, pbc, p, [types[i - 1] for i in at])
+            dE = np.abs(r["energies"] - ref["energies"]).max()
+            dF = np.abs(r["forces"] - ref["forces"]).max()
+            print("max |dE| %.3e eV/atom  max |dF| %.3e eV/A  (|F| up to %.2f)" % (dE, dF, np.abs(ref["forces"]).max()))
+            assert np.abs(ref["forces"]).max() > 0.1
+            assert dE < E_TOL
+            assert dF < F_TOL
+    finally:
+        ann.release()
