@@ -76,6 +76,9 @@ SIGNATURES = {
     "aenet_amd_batch_eval_model": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
     "aenet_amd_batch_autotune": (None, [_vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _pint, _pint, _vp, _pint]),
     "aenet_amd_batch_set_launch": (None, [_vp, _vp, C.c_int, C.c_int, C.c_int, _pint]),
+    "aenet_amd_batch_set_buckets": (None, [_vp, _vp, C.c_int, _ip, _ip, _pint]),
+    "aenet_amd_batch_buckets": (None, [_vp, _vp, _pint, _vp, _vp, _vp, _vp, _vp, _pint]),
+    "aenet_amd_bucket_caps": (None, [_ip, C.c_int, C.c_int, _ip, _pint]),
     "aenet_amd_batch_eval_active": (None, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, C.c_int, _vp, _vp, _pint]),
     "aenet_amd_batch_vjp": (None, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _pint]),
     "aenet_amd_batch_descriptors_model": (None, [_vp, _vp, _vp, _vp, C.c_int, _vp, C.c_int, _vp, _pint]),

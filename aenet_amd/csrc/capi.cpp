@@ -1180,6 +1180,49 @@ void aenet_amd_batch_set_launch(aenet_amd_batch *batch, const aenet_amd_model *m
   if (*stat) set_err(*stat, "aenet_amd_batch_set_launch: " + err);
 }
 
+void aenet_amd_batch_set_buckets(aenet_amd_batch *batch, const aenet_amd_model *model, int n, const int caps[],
+                                 const int shapes[], int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!batch || !dm) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or no model");
+    return;
+  }
+  std::string err;
+  *stat = batch->batch.set_buckets(*dm, n, caps, shapes, err);
+  if (*stat) set_err(*stat, "aenet_amd_batch_set_buckets: " + err);
+}
+
+void aenet_amd_batch_buckets(aenet_amd_batch *batch, const aenet_amd_model *model, int *n, int caps[],
+                             int shapes[], int atoms[], double ms[], double totals[2], int *stat) {
+  const DeviceModel *dm = pick_model(model);
+  if (!batch || !dm || !n) {
+    *stat = set_err(AENET_STATUS_ERR_INIT, "no batch or no model");
+    return;
+  }
+  Batch::BucketPlan pl;
+  batch->batch.buckets(*dm, pl);
+  *n = pl.n;
+  for (int k = 0; k < pl.n; ++k) {
+    if (caps) caps[k] = pl.cap[k];
+    if (shapes)
+      for (int c = 0; c < 3; ++c) shapes[3 * k + c] = pl.shape[k][c];
+    if (atoms) atoms[k] = pl.atoms[k];
+    if (ms) ms[k] = pl.ms[k];
+  }
+  if (totals) {
+    totals[0] = pl.ms_single;
+    totals[1] = pl.ms_buckets;
+  }
+  *stat = AENET_STATUS_OK;
+}
+
+void aenet_amd_bucket_caps(const int counts[], int natoms, int nb_cap, int caps[], int *ncaps) {
+  std::vector<int> c(counts, counts + (natoms > 0 ? natoms : 0));
+  const std::vector<int> r = bucket_caps_for(c, nb_cap);
+  *ncaps = (int)r.size();
+  for (size_t k = 0; k < r.size(); ++k) caps[k] = r[k];
+}
+
 void aenet_amd_batch_eval_active(aenet_amd_batch *batch, const aenet_amd_model *model, const double *d_coords,
                                  const int *d_types, const int *d_active, const int *d_nactive, int max_active,
                                  int reuse_lists, double *d_E_atoms, void *stream, int *stat) {

@@ -334,22 +334,7 @@ int Batch::eval(const DeviceModel &dm, const double *d_coords, const int *d_type
   std::memset(&o, 0, sizeof(o));
   o.E_atom = d_E_atom ? d_E_atom : scratch_E_.p;
   o.F = d_F;
-  EnvDev e;
-  std::memset(&e, 0, sizeof(e));
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (timing_ && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
-    (void)hipEventRecord(e0, stream);
-  }
-  if (dm.desc == DESC_CHEBYSHEV)
-    launch_atom_cheb(dm.max_rN, dm.max_aN, 0, d_F != nullptr, sizes_for(dm, b.natoms), md, b, nb, o, e,
-                     stream);
-  else
-    launch_atom_bp(0, d_F != nullptr, sizes_for(dm, b.natoms), md, b, nb, o, e, stream);
-  if (e0 && e1) {
-    (void)hipEventRecord(e1, stream);
-    ev_.push_back(e0);
-    ev_.push_back(e1);
-  }
+  launch_atoms(dm, d_F != nullptr, md, b, nb, o, stream);
   // E_struct = [E_tot(0..ns-1), E_coh(0..ns-1)]; the forces into FP64
   launch_struct_energy(b, md, o.E_atom, d_E_struct, d_E_struct ? d_E_struct + b.nstruct : nullptr, d_F, 1.0,
                        stream);
@@ -467,6 +452,254 @@ int Batch::descriptors(const DeviceModel &dm, const double *d_coords, const int 
   return hipGetLastError() == hipSuccess ? AENET_STATUS_OK : AENET_STATUS_ERR_DEVICE;
 }
 
+// events around [begin, end) of a launch sequence while timing
+struct EventPair {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  void begin(bool on, hipStream_t s) {
+    if (on && hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess)
+      (void)hipEventRecord(e0, s);
+  }
+  void end(hipStream_t s, std::vector<hipEvent_t> &ev) {
+    if (e0 && e1) {
+      (void)hipEventRecord(e1, s);
+      ev.push_back(e0);
+      ev.push_back(e1);
+    }
+    e0 = e1 = nullptr;
+  }
+};
+
+void Batch::launch_atoms(const DeviceModel &dm, bool forces, const ModelDev &md, const BatchDev &b, const NbDev &nb,
+                         const OutDev &o, hipStream_t stream) {
+  EnvDev e;
+  std::memset(&e, 0, sizeof(e));
+  const auto it = buckets_.find(&dm);
+  if (it == buckets_.end() || it->second.n < 2) {
+    EventPair t;
+    t.begin(timing_, stream);
+    if (dm.desc == DESC_CHEBYSHEV)
+      launch_atom_cheb(dm.max_rN, dm.max_aN, 0, forces, sizes_for(dm, b.natoms), md, b, nb, o, e, stream);
+    else
+      launch_atom_bp(0, forces, sizes_for(dm, b.natoms), md, b, nb, o, e, stream);
+    t.end(stream, ev_);
+    return;
+  }
+  // one launch per neighbor-count bucket, each over the device list of its
+  // atoms, with LDS regions of the bucket's capacity and its own shape
+  const BucketPlan &pl = it->second;
+  BucketCaps bc;
+  bc.n = pl.n;
+  for (int k = 0; k < pl.n; ++k) bc.cap[k] = std::min(pl.cap[k], nb_cap_);
+  EventPair all;
+  all.begin(timing_ && timed_bucket_ < 0, stream);
+  launch_bucket_atoms(nb.count, b.natoms, bc, bucket_lists_.p, bucket_counts_.p, stream);
+  for (int k = 0; k < pl.n; ++k) {
+    // the persistent grid sized for the atoms the bucket held when measured,
+    // with room (any count is evaluated, in more rounds)
+    const int expect = pl.atoms[k] < 0 ? b.natoms : std::min(b.natoms, pl.atoms[k] + pl.atoms[k] / 4 + 256);
+    AtomSizes z = dm.sizes(expect, bc.cap[k], bucket_lists_.p + (size_t)k * b.natoms, bucket_counts_.p + k);
+    z.tune_waves = pl.shape[k][0];
+    z.tune_staged = pl.shape[k][1];
+    z.tune_tile = pl.shape[k][2];
+    EventPair one;
+    one.begin(timing_ && timed_bucket_ == k, stream);
+    if (dm.desc == DESC_CHEBYSHEV)
+      launch_atom_cheb(dm.max_rN, dm.max_aN, 0, forces, z, md, b, nb, o, e, stream);
+    else
+      launch_atom_bp(0, forces, z, md, b, nb, o, e, stream);
+    one.end(stream, ev_);
+  }
+  all.end(stream, ev_);
+}
+
+// Bucket caps for a histogram of neighbor counts: powers of two from 16
+// below the capacity, then the capacity; a bucket with fewer than
+// max(natoms / 32, 2048) atoms joins the next larger one (the last bucket
+// always stays).  One cap: no buckets.
+std::vector<int> bucket_caps_for(const std::vector<int> &counts, int nb_cap) {
+  if (nb_cap < 1) return {};
+  std::vector<int> cand;
+  for (int c = 16; c < nb_cap && (int)cand.size() < MAX_BUCKETS - 1; c *= 2) cand.push_back(c);
+  cand.push_back(nb_cap);
+  std::vector<long> in(cand.size(), 0);
+  for (int n : counts) {
+    size_t k = 0;
+    while (k + 1 < cand.size() && n > cand[k]) ++k;
+    ++in[k];
+  }
+  const long minimum = std::max<long>((long)counts.size() / 32, 2048);
+  std::vector<int> caps;
+  long carry = 0;
+  for (size_t k = 0; k < cand.size(); ++k) {
+    carry += in[k];
+    if (k + 1 == cand.size() || carry >= minimum) {
+      caps.push_back(cand[k]);
+      carry = 0;
+    }
+  }
+  return caps;
+}
+
+int Batch::set_buckets(const DeviceModel &dm, int n, const int *caps, const int *shapes, std::string &err) {
+  if (n == 0) {
+    buckets_.erase(&dm);
+    return AENET_STATUS_OK;
+  }
+  if (n < 0 || n > MAX_BUCKETS || !caps || !shapes) {
+    err = "bucket count out of range";
+    return AENET_AMD_ERR_ARG_VALUE;
+  }
+  const bool cheb = dm.desc == DESC_CHEBYSHEV;
+  const int max_waves = cheb ? CHEB_MAX_WAVES : ATOM_MAX_WAVES;
+  BucketPlan pl;
+  pl.n = n;
+  for (int k = 0; k < n; ++k) pl.atoms[k] = -1;
+  for (int k = 0; k < n; ++k) {
+    if (caps[k] < 1 || (k > 0 && caps[k] <= caps[k - 1]) || (k + 1 < n && caps[k] >= nb_cap_)) {
+      err = "bucket caps must ascend and stay below the batch's neighbor capacity but for the last";
+      return AENET_AMD_ERR_ARG_VALUE;
+    }
+    pl.cap[k] = k + 1 < n ? caps[k] : nb_cap_;
+    const int waves = shapes[3 * k], staged = shapes[3 * k + 1], tile = shapes[3 * k + 2];
+    if (waves < 1 || waves > max_waves || (waves & (waves - 1)) || staged < 0 || staged > 1 || tile < 0 ||
+        tile > 1 || (tile && !(cheb && dm.net_uniform))) {
+      err = "bucket launch shape out of range";
+      return AENET_AMD_ERR_ARG_VALUE;
+    }
+    AtomSizes z = dm.sizes(plan_.natoms, pl.cap[k]);
+    z.tune_waves = waves;
+    z.tune_staged = staged;
+    z.tune_tile = tile;
+    if (!(cheb ? atom_cheb_shape_ok(z) : atom_bp_shape_ok(z))) {
+      err = "bucket launch shape does not fit the device";
+      return AENET_AMD_ERR_ARG_VALUE;
+    }
+    pl.shape[k] = {waves, staged, tile};
+  }
+  if (bucket_lists_.n < (size_t)MAX_BUCKETS * plan_.natoms &&
+      bucket_lists_.alloc((size_t)MAX_BUCKETS * std::max(plan_.natoms, 1)))
+    return AENET_STATUS_ERR_MALLOC;
+  if (bucket_counts_.n < (size_t)MAX_BUCKETS && bucket_counts_.alloc(MAX_BUCKETS)) return AENET_STATUS_ERR_MALLOC;
+  buckets_[&dm] = pl;
+  return AENET_STATUS_OK;
+}
+
+void Batch::buckets(const DeviceModel &dm, BucketPlan &out) const {
+  const auto it = buckets_.find(&dm);
+  out = it == buckets_.end() ? BucketPlan() : it->second;
+}
+
+int Batch::tune_buckets(const DeviceModel &dm, const double *d_coords, const int *d_types, bool forces, int repeats,
+                        double ms_single, hipStream_t stream, std::string &err) {
+  buckets_.erase(&dm);
+  const int nat = plan_.natoms;
+  if (nat < 4096 || nb_cap_ <= 16) return AENET_STATUS_OK;
+  DBuf<double> E_struct, F;
+  if (E_struct.alloc(2 * (size_t)plan_.st.size()) || (forces && F.alloc(3 * (size_t)nat))) {
+    err = "device allocation failed";
+    return AENET_STATUS_ERR_MALLOC;
+  }
+  double* const Fp = forces ? F.p : nullptr;
+  // the neighbor counts of these coordinates
+  int st = eval(dm, d_coords, d_types, scratch_E_.p, E_struct.p, Fp, stream);
+  std::vector<int> cnt(nat);
+  if (!st && (hipStreamSynchronize(stream) != hipSuccess ||
+              hipMemcpy(cnt.data(), nb_count_.p, nat * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess))
+    st = AENET_STATUS_ERR_DEVICE;
+  if (st) {
+    err = "evaluation failed while tuning";
+    return st;
+  }
+  const std::vector<int> caps = bucket_caps_for(cnt, nb_cap_);
+  if (caps.size() < 2) return AENET_STATUS_OK;
+  const auto single = tune_.find(&dm);
+  std::vector<int> shapes;
+  for (size_t k = 0; k < caps.size(); ++k) {
+    shapes.push_back(single != tune_.end() ? single->second[0] : 1);
+    shapes.push_back(single != tune_.end() ? std::max(single->second[1], 0) : 0);
+    shapes.push_back(0);
+  }
+  // the single launch's shape may not fit a bucket (it does: a smaller
+  // capacity needs less LDS); any failure drops the plan
+  st = set_buckets(dm, (int)caps.size(), caps.data(), shapes.data(), err);
+  if (st) {
+    buckets_.erase(&dm);
+    return st == AENET_STATUS_ERR_MALLOC ? st : AENET_STATUS_OK;
+  }
+  BucketPlan &pl = buckets_[&dm];
+  for (int k = 0; k < pl.n; ++k) {
+    long a = 0;
+    for (int n : cnt) {
+      int b = pl.n - 1;
+      for (int j = pl.n - 2; j >= 0; --j)
+        if (n <= pl.cap[j]) b = j;
+      a += b == k;
+    }
+    pl.atoms[k] = (int)a;
+  }
+  const bool was_timing = timing_;
+  const bool cheb = dm.desc == DESC_CHEBYSHEV;
+  const int max_waves = cheb ? CHEB_MAX_WAVES : ATOM_MAX_WAVES;
+  const int max_tile = cheb && dm.net_uniform ? 1 : 0;
+  auto timed = [&](double &avg) -> int {
+    double t0 = 0.0;
+    int c0 = 0;
+    (void)kernel_time(t0, c0);
+    timing_ = false;
+    int s = eval(dm, d_coords, d_types, scratch_E_.p, E_struct.p, Fp, stream);  // warm-up
+    timing_ = true;
+    for (int r = 0; r < repeats && !s; ++r) s = eval(dm, d_coords, d_types, scratch_E_.p, E_struct.p, Fp, stream);
+    timing_ = false;
+    double ms = 0.0;
+    int count = 0;
+    if (!s) s = kernel_time(ms, count);
+    avg = count > 0 ? ms / count : 1e300;
+    return s;
+  };
+  for (int k = 0; k < pl.n && !st; ++k) {
+    timed_bucket_ = k;
+    double best = 1e300;
+    std::array<int, 3> keep = pl.shape[k];
+    for (int tile = 0; tile <= max_tile && !st; ++tile)
+      for (int waves = 1; waves <= max_waves && !st; waves *= 2)
+        for (int staged = 0; staged <= 1 && !st; ++staged) {
+          AtomSizes z = dm.sizes(nat, pl.cap[k]);
+          z.tune_waves = waves;
+          z.tune_staged = staged;
+          z.tune_tile = tile;
+          if (!(cheb ? atom_cheb_shape_ok(z) : atom_bp_shape_ok(z))) continue;
+          pl.shape[k] = {waves, staged, tile};
+          double avg = 0.0;
+          st = timed(avg);
+          if (!st && avg < best) {
+            best = avg;
+            keep = pl.shape[k];
+          }
+        }
+    pl.shape[k] = keep;
+    pl.ms[k] = best < 1e300 ? best : 0.0;
+  }
+  timed_bucket_ = -1;
+  double total = 1e300;
+  if (!st) st = timed(total);
+  timing_ = was_timing;
+  {
+    double t1 = 0.0;
+    int c1 = 0;
+    (void)search_time(t1, c1);  // the tuner's own search launches are not reported
+  }
+  if (st) {
+    buckets_.erase(&dm);
+    err = "evaluation failed while tuning";
+    return st;
+  }
+  pl.ms_single = ms_single;
+  pl.ms_buckets = total;
+  // buckets only where they measure faster than the single launch
+  if (!(total < 0.98 * ms_single)) buckets_.erase(&dm);
+  return AENET_STATUS_OK;
+}
+
 AtomSizes Batch::sizes_for(const DeviceModel &dm, int natoms, const int *active, const int *nactive) const {
   AtomSizes z = dm.sizes(natoms, nb_cap_, active, nactive);
   const auto it = tune_.find(&dm);
@@ -493,6 +726,7 @@ int Batch::autotune(const DeviceModel &dm, const double *d_coords, const int *d_
   int c0 = 0;
   (void)kernel_time(t0, c0);  // drop events of earlier evaluations
   tune_.erase(&dm);
+  buckets_.erase(&dm);
   const int max_waves = dm.desc == DESC_CHEBYSHEV ? CHEB_MAX_WAVES : ATOM_MAX_WAVES;
   // the matrix-tile network is a candidate where the Chebyshev kernel has it
   const int max_tile = dm.desc == DESC_CHEBYSHEV && dm.net_uniform ? 1 : 0;
@@ -549,10 +783,12 @@ int Batch::autotune(const DeviceModel &dm, const double *d_coords, const int *d_
     return AENET_STATUS_ERR_DEVICE;
   }
   tune_[&dm] = choice;
-  return AENET_STATUS_OK;
+  // then neighbor-count buckets, kept where they beat the single launch
+  return tune_buckets(dm, d_coords, d_types, forces, repeats, best, stream, err);
 }
 
 int Batch::set_launch(const DeviceModel &dm, int waves, int staged, int tile, std::string &err) {
+  buckets_.erase(&dm);
   if (waves == 0) {
     tune_.erase(&dm);
     return AENET_STATUS_OK;

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "aenet_amd_internal.h"
+#include "kernels.h"
 #include "model.h"
 #include "plan.h"
 
@@ -98,6 +99,10 @@ struct DeviceModel {
   ModelDev view() const;
 };
 
+// neighbor-count bucket caps for the neighbor counts of a batch's atoms
+// (the last cap is nb_cap; one cap: no buckets)
+std::vector<int> bucket_caps_for(const std::vector<int> &counts, int nb_cap);
+
 // a fixed set of structures (shapes, lattices, atom counts); coordinates
 // and types may change between evaluations
 class Batch {
@@ -144,8 +149,25 @@ class Batch {
   int autotune(const DeviceModel &dm, const double *d_coords, const int *d_types, bool forces, int repeats,
                hipStream_t stream, std::vector<std::array<double, 4>> &out, std::string &err);
   // a launch shape given by the caller (as one the tuner keeps); waves = 0
-  // returns to the occupancy model
+  // returns to the occupancy model.  Drops a bucket plan of dm.
   int set_launch(const DeviceModel &dm, int waves, int staged, int tile, std::string &err);
+  // Neighbor-count buckets of the atom kernel for dm: n caps (ascending,
+  // the last one at least the batch's neighbor capacity, which it becomes)
+  // and a launch shape (waves, staged, tile) per bucket; n = 0 drops the
+  // plan.  The tuner makes one where it measures faster than one launch.
+  int set_buckets(const DeviceModel &dm, int n, const int *caps, const int *shapes, std::string &err);
+  // the plan in use for dm: n (0: none), caps, shapes (3 per bucket) and,
+  // when the tuner made it, each bucket's measured atom-kernel time and
+  // the totals with and without buckets (ms; 0: not measured)
+  struct BucketPlan {
+    int n = 0;
+    int cap[MAX_BUCKETS] = {};
+    std::array<int, 3> shape[MAX_BUCKETS] = {};
+    double ms[MAX_BUCKETS] = {};
+    int atoms[MAX_BUCKETS] = {};  // atoms per bucket when measured (-1: unknown)
+    double ms_single = 0.0, ms_buckets = 0.0;
+  };
+  void buckets(const DeviceModel &dm, BucketPlan &out) const;
   // HIP events around the atom kernel of every evaluation while enabled
   void set_timing(bool on);
   // summed duration (ms) and count of the timed atom-kernel launches since
@@ -170,6 +192,15 @@ class Batch {
   AtomSizes sizes_for(const DeviceModel &dm, int natoms, const int *active = nullptr,
                       const int *nactive = nullptr) const;
   std::map<const DeviceModel *, std::array<int, 3>> tune_;  // model -> (waves, staged, tile network)
+  std::map<const DeviceModel *, BucketPlan> buckets_;     // model -> neighbor-count buckets
+  DBuf<int> bucket_lists_, bucket_counts_;                // [MAX_BUCKETS * natoms], [MAX_BUCKETS]
+  int timed_bucket_ = -1;                                 // events around one bucket's launch only (tuner)
+  // the atom kernel(s) of a full evaluation: one launch, or the bucket
+  // launches of dm's plan
+  void launch_atoms(const DeviceModel &dm, bool forces, const ModelDev &md, const BatchDev &b, const NbDev &nb,
+                    const OutDev &o, hipStream_t stream);
+  int tune_buckets(const DeviceModel &dm, const double *d_coords, const int *d_types, bool forces, int repeats,
+                   double ms_single, hipStream_t stream, std::string &err);
   int allocate(std::string &err);
   PlanHost plan_;
   double rc_max_ = 0.0;

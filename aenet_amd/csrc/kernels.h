@@ -29,8 +29,23 @@ constexpr int FORCE_SLOT_MAX_ATOMS = 16384;
 constexpr int BP_TRIP_CAP = 4096;
 constexpr int BP_TRIP_WAVES_PER_CU = 16;
 
+// neighbor-count buckets of a batch's atoms: bucket k holds the atoms whose
+// neighbor count n has cap[k-1] < n <= cap[k] (the last cap is the batch's
+// capacity), each bucket evaluated by its own atom-kernel launch with LDS
+// regions of its cap and its own measured launch shape
+constexpr int MAX_BUCKETS = 6;
+struct BucketCaps {
+  int n;
+  int cap[MAX_BUCKETS];
+};
+
 // kernels.hip
 void launch_fill_i32(int *p, int v, long n, hipStream_t s);
+// the atoms into their buckets: lists[k * natoms ..] and counts[k] (cleared
+// here first); within a bucket, runs of 256 consecutive atoms keep their
+// order
+void launch_bucket_atoms(const int *nb_count, int natoms, const BucketCaps &bc, int *lists, int *counts,
+                         hipStream_t s);
 void launch_fill_f64(double *p, double v, long n, hipStream_t s);
 void launch_scale_f64(double *p, double a, long n, hipStream_t s);
 // (cell, image) combinations of every home cell; once per batch

@@ -889,6 +889,48 @@ void launch_combos(const BatchDev &b, int *combo_count, int *combos, hipStream_t
   if (b.ncells > 0) hipLaunchKernelGGL(k_combos, dim3(b.ncells), dim3(256), 0, s, b, combo_count, combos);
 }
 
+// Neighbor-count buckets.  Thread per atom; within a workgroup the atoms
+// of a bucket are ranked in atom order (wavefront ballots), and one global
+// atomic per bucket and workgroup reserves their places.  The order of the
+// workgroups' runs in a list is the order of those atomics; the atom
+// kernels' results do not depend on it.
+__global__ __launch_bounds__(256) void k_bucket_atoms(const int *nb_count, int natoms, BucketCaps bc, int *lists,
+                                                      int *counts) {
+  __shared__ int wcnt[4][MAX_BUCKETS];
+  __shared__ int base[MAX_BUCKETS];
+  const int tid = threadIdx.x, w = tid / WAVE, lane = lane_id();
+  const int g = blockIdx.x * 256 + tid;
+  const bool in = g < natoms;
+  const int n = in ? nb_count[g] : 0;
+  int bk = bc.n - 1;
+  for (int k = bc.n - 2; k >= 0; --k)
+    if (n <= bc.cap[k]) bk = k;
+  int rank = 0;
+  for (int k = 0; k < bc.n; ++k) {
+    const unsigned long long m = wave_ballot(in && bk == k);
+    if (in && bk == k) rank = mask_prefix(m);
+    if (lane == 0) wcnt[w][k] = popc64(m);
+  }
+  __syncthreads();
+  if (tid < bc.n) {
+    const int t = wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+    base[tid] = t ? atomicAdd(&counts[tid], t) : 0;
+  }
+  __syncthreads();
+  if (in) {
+    int off = base[bk] + rank;
+    for (int v = 0; v < w; ++v) off += wcnt[v][bk];
+    lists[(size_t)bk * natoms + off] = g;
+  }
+}
+
+void launch_bucket_atoms(const int *nb_count, int natoms, const BucketCaps &bc, int *lists, int *counts,
+                         hipStream_t s) {
+  launch_fill_i32(counts, 0, bc.n, s);
+  if (natoms > 0) hipLaunchKernelGGL(k_bucket_atoms, dim3(nblk(natoms, 256)), dim3(256), 0, s, nb_count, natoms, bc,
+                                     lists, counts);
+}
+
 void launch_binning(const BatchDev &b, int *tmp, double *F, hipStream_t s) {
   if (b.natoms <= SMALL_ATOMS && b.ncells <= SMALL_CELLS) {
     hipLaunchKernelGGL(k_frac, dim3(nblk(b.natoms, 256)), dim3(256), 0, s, b, F);

@@ -209,6 +209,46 @@ class DeviceBatch:
                                                  C.byref(stat))
         check_status(stat.value, "aenet_amd_batch_set_launch")
 
+    def set_buckets(self, caps, shapes, model=None):
+        """Neighbor-count buckets of the atom kernel (aenet_amd_batch_set_buckets):
+        caps ascending (the last becomes the batch's neighbor capacity), one
+        (waves, staged, tile) shape per bucket; caps = [] drops the plan."""
+        caps = np.ascontiguousarray(caps, dtype=np.int32)
+        sh = np.ascontiguousarray(np.asarray(shapes, dtype=np.int32).reshape(-1))
+        if len(caps) == 0:
+            caps = np.zeros(1, dtype=np.int32)
+            sh = np.zeros(3, dtype=np.int32)
+            n = 0
+        else:
+            n = len(caps)
+            if sh.size != 3 * n:
+                raise ValueError("one (waves, staged, tile) shape per bucket")
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_set_buckets(self._h, self._model(model), n, caps, sh, C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_set_buckets")
+
+    def buckets(self, model=None):
+        """The bucket plan in use (aenet_amd_batch_buckets): a dict with caps,
+        shapes, atoms and ms per bucket and the tuner's totals (ms) for one
+        launch and for the buckets; None without a plan."""
+        n = C.c_int(0)
+        caps = np.zeros(6, dtype=np.int32)
+        shapes = np.zeros(18, dtype=np.int32)
+        atoms = np.zeros(6, dtype=np.int32)
+        ms = np.zeros(6)
+        tot = np.zeros(2)
+        stat = C.c_int(0)
+        _native.lib().aenet_amd_batch_buckets(self._h, self._model(model), C.byref(n), caps.ctypes.data,
+                                              shapes.ctypes.data, atoms.ctypes.data, ms.ctypes.data,
+                                              tot.ctypes.data, C.byref(stat))
+        check_status(stat.value, "aenet_amd_batch_buckets")
+        k = n.value
+        if k == 0:
+            return None
+        return {"caps": caps[:k].tolist(), "shapes": shapes[:3 * k].reshape(k, 3).tolist(),
+                "atoms": atoms[:k].tolist(), "ms": ms[:k].tolist(), "ms_single": float(tot[0]),
+                "ms_buckets": float(tot[1])}
+
     def descriptors(self, stride, scaled=False, coords=None, types=None, model=None):
         coords = self.coords if coords is None else coords
         types = self.types if types is None else types

@@ -220,6 +220,19 @@ def batched_bp(steps, warmup):
             "frac_fp32_peak": flops / k_s / FP32_PEAK,
             "launch_shape": launch_shape(chosen, rows),
         }
+        plan = db.buckets()
+        if plan is not None:
+            # kernel_ms then covers the bucketing launch and one atom-kernel
+            # launch per neighbor-count bucket
+            out["kernel"] = "k_bucket_atoms + k_atom_bp<0,true> per bucket"
+            out["neighbor_count_buckets"] = {
+                "caps": plan["caps"], "atoms": plan["atoms"],
+                "shapes": [{"waves_per_workgroup": w, "weights_in_lds": st, "tile_network": t}
+                           for (w, st, t) in plan["shapes"]],
+                "bucket_kernel_us": [m * 1e3 for m in plan["ms"]],
+                "tuner_single_launch_us": plan["ms_single"] * 1e3, "tuner_buckets_us": plan["ms_buckets"] * 1e3}
+        else:
+            out["neighbor_count_buckets"] = None
         del db
         return out
     finally:

@@ -134,6 +134,41 @@ void aenet_amd_batch_autotune(aenet_amd_batch *batch, const aenet_amd_model *mod
 void aenet_amd_batch_set_launch(aenet_amd_batch *batch, const aenet_amd_model *model,
                                 int waves, int staged, int tile, int *stat);
 
+/* Neighbor-count buckets of the atom kernel for this batch and model: the
+ * atoms whose neighbor count n has caps[k-1] < n <= caps[k] are evaluated
+ * by launch k, with LDS regions sized for caps[k] and the launch shape
+ * shapes[3k .. 3k+2] = (waves, staged, tile) as in aenet_amd_batch_set_launch.
+ * caps ascend; all but the last stay below the batch's neighbor capacity,
+ * and the last becomes that capacity.  Each evaluation puts the atoms into
+ * their buckets on the device (one small launch), so the counts may change
+ * between evaluations.  n = 0 drops the plan; n <= 6.  The results equal
+ * those of a single launch (per-atom energies and the 64-bit fixed-point
+ * force sums do not depend on the launch shape without the tile network).
+ * aenet_amd_batch_autotune makes such a plan itself, after timing the
+ * single launch, where the batch has at least 4096 atoms, its neighbor
+ * counts fill more than one bucket, and the buckets measure at least 2 %
+ * faster.  aenet_amd_batch_set_launch drops the plan.  No counterpart in
+ * the reference. */
+void aenet_amd_batch_set_buckets(aenet_amd_batch *batch, const aenet_amd_model *model,
+                                 int n, const int caps[], const int shapes[], int *stat);
+
+/* The bucket plan in use: *n (0: none), caps[n], shapes[3n], atoms[n]
+ * (atoms per bucket when the tuner measured), ms[n] (each
+ * bucket's measured atom-kernel time) and totals = (single launch, bucket
+ * launches with the bucketing kernel) in ms as the tuner measured them
+ * (atoms -1 and times 0: set by the caller).  Arrays may be NULL; room for 6 buckets.  No
+ * counterpart in the reference. */
+void aenet_amd_batch_buckets(aenet_amd_batch *batch, const aenet_amd_model *model, int *n,
+                             int caps[], int shapes[], int atoms[], double ms[], double totals[2],
+                             int *stat);
+
+/* The bucket caps the autotuner would take for these neighbor counts
+ * (host arrays, no device needed): powers of two from 16 below nb_cap,
+ * then nb_cap; a bucket with fewer than max(natoms/32, 2048) atoms joins
+ * the next larger one.  *ncaps = 1: no buckets.  No counterpart in the
+ * reference. */
+void aenet_amd_bucket_caps(const int counts[], int natoms, int nb_cap, int caps[], int *ncaps);
+
 /* Bit-exact neighbor lists (lcl_nbdist_cart, src/ext/lclist.f90:520-677)
  * of every atom of the batch, device pointers, capacity cap per atom:
  * d_count[natoms], d_coo[natoms*cap*3], d_dist/d_list/d_type[natoms*cap];

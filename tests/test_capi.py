@@ -139,3 +139,36 @@ def test_python_mirror_without_device():
     assert st.value == 0
     _native.lib().aenet_final(C.byref(st))
     assert InitError is not None
+
+
+def test_bucket_caps():
+    """Neighbor-count buckets the autotuner would take (aenet_amd_bucket_caps,
+    host only): powers of two from 16 below the capacity, then the
+    capacity; a bucket with fewer than max(natoms/32, 2048) atoms joins the
+    next larger one, and the last bucket always stays."""
+    import numpy as np
+    lib = _native.lib()
+
+    def caps(counts, cap):
+        c = np.ascontiguousarray(counts, dtype=np.int32)
+        out = np.zeros(6, dtype=np.int32)
+        n = C.c_int(0)
+        lib.aenet_amd_bucket_caps(c, len(c), cap, out, C.byref(n))
+        return out[:n.value].tolist()
+
+    # clusters (10 neighbors) and bulk (80): a bucket each, the empty last stays
+    assert caps([10] * 3000 + [80] * 5000, 160) == [16, 128, 160]
+    # too few small atoms for a bucket of their own: they join the next one
+    assert caps([10] * 1000 + [80] * 5000, 160) == [128, 160]
+    # one neighbor count: one bucket up to it, then the capacity
+    assert caps([80] * 8000, 128) == [128]
+    assert caps([40] * 8000, 128) == [64, 128]
+    # every count above the last power of two: one bucket
+    assert caps([150] * 8000, 160) == [160]
+    # the share rule: 1/32 of 200 000 atoms is 6250
+    assert caps([20] * 6000 + [60] * 194000, 512) == [64, 512]
+    assert caps([20] * 7000 + [60] * 193000, 512) == [32, 64, 512]
+    # spread over every size: the small ones merged, at most six buckets
+    assert caps(list(range(1, 513)) * 40, 512) == [64, 128, 256, 512]
+    assert caps(list(range(1, 513)) * 200, 512) == [16, 32, 64, 128, 256, 512]
+    assert caps([], 64) == [64]

@@ -246,6 +246,73 @@ def test_launch_shapes_and_tile_network(device):
         ann.release()
 
 
+def test_neighbor_count_buckets(device, structures):
+    """Neighbor-count buckets (aenet_amd_batch_set_buckets): every bucket's
+    atoms evaluated by a launch of their own, with LDS regions of the
+    bucket's cap and their own launch shape, give bitwise the energies and
+    forces of the single launch (per-wavefront network; per-atom results
+    and 64-bit fixed-point force sums do not depend on the launch).  For
+    the 2048-structure Behler-Parrinello batch (clusters of 8 atoms to
+    512-atom bulk) and an uneven two-species Chebyshev batch, with an empty
+    bucket, and with other coordinates evaluated in between so that values
+    a launch fails to write would show.  The tuner's plan (kept only where
+    it measures faster) reproduces them too."""
+    import torch
+    from aenet_amd import synth
+    from aenet_amd.core import ANNPotentials
+    from aenet_amd.device import DeviceBatch
+    cases = ((pot_files("bp3", NAMES), structures, [(1, 0, 0), (4, 0, 0), (2, 1, 0), (4, 1, 0)]),
+             (pot_files("alloy", ["A", "B"]), synth.batch_uneven(400, ntypes=2),
+              [(16, 1, 0), (4, 0, 0), (8, 1, 0), (1, 0, 0)]))
+    for pots, structs, shapes in cases:
+        ann = ANNPotentials(pots)
+        try:
+            db = DeviceBatch(structs, device=device)
+            _, _, E0, F0 = db.eval()
+            E0, F0 = E0.clone(), F0.clone()
+            other = db.coords + 0.03 * torch.sin(torch.arange(db.coords.numel(), device=db.coords.device,
+                                                              dtype=torch.float64)).reshape(db.coords.shape)
+            for caps in ([16, 32, 64, 512], [8, 512], [24, 40, 48, 56, 60, 512]):
+                sh = [shapes[k % len(shapes)] for k in range(len(caps))]
+                print("buckets", db.natoms, caps, sh, flush=True)
+                db.set_buckets(caps, sh)
+                plan = db.buckets()
+                assert plan["caps"][:-1] == caps[:-1] and plan["caps"][-1] > caps[-2]
+                assert plan["atoms"] == [-1] * len(caps)
+                _, _, Eo, _ = db.eval(coords=other)
+                assert (Eo - E0).abs().max().item() > 1e-4
+                _, _, E1, F1 = db.eval()
+                assert torch.equal(E0, E1)
+                assert torch.equal(F0, F1)
+            db.set_buckets([], [])
+            assert db.buckets() is None
+            _, _, E2, F2 = db.eval()
+            assert torch.equal(E0, E2) and torch.equal(F0, F2)
+            with pytest.raises(Exception):
+                db.set_buckets([32, 16, 512], [(1, 0, 0)] * 3)
+            with pytest.raises(Exception):
+                db.set_buckets([16, 512], [(3, 0, 0), (1, 0, 0)])
+            chosen, _ = db.autotune(forces=True, repeats=1)
+            plan = db.buckets()
+            print("tuner", chosen, plan, flush=True)
+            _, _, E3, F3 = db.eval()
+            tile = chosen[2] == 1 or (plan is not None and any(s[2] for s in plan["shapes"]))
+            if plan is not None:
+                assert len(plan["caps"]) >= 2 and sum(plan["atoms"]) == db.natoms
+                assert plan["ms_buckets"] < plan["ms_single"]
+            if tile:
+                assert (E3 - E0).abs().max().item() < 2 * E_TOL
+                assert (F3 - F0).abs().max().item() < 2 * F_TOL
+            else:
+                assert torch.equal(E0, E3) and torch.equal(F0, F3)
+            # a launch shape set by the caller drops the plan
+            db.set_launch(1, 0, 0)
+            assert db.buckets() is None
+            del db
+        finally:
+            ann.release()
+
+
 def test_forces_bitwise_reproducible(device, structures):
     """Forces are summed as 64-bit fixed-point integers, so the order in
     which the atomics of different atoms arrive changes nothing: repeated
