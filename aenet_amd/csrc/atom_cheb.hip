@@ -684,7 +684,9 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
     const int groups = na >= 2 ? WAVE / na : 1;
     const int rot_iters = na >= 2 ? (rounds + groups - 1) / groups : 0;
     if (use_aux && na >= 2 && rot_iters <= 2 * ((ntrip + WAVE - 1) / WAVE)) {
-      const int h = lane / na, a = lane - h * na;
+      // lane / na from a float reciprocal: (lane + 1/2) / na stays at least
+      // 1/128 from an integer (lane < 64, na <= 64), far beyond the rounding
+      const int h = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)na)), a = lane - h * na;
       float acc[3] = {0.0f, 0.0f, 0.0f};
       for (int it = 0; it < rot_iters; ++it) {
         const int d = it * groups + h + 1;

@@ -234,10 +234,13 @@ __device__ inline float network(const ModelDev &md, const SpeciesDev &sp, WPtr W
     // lanes = (node, part): with NP the node count rounded up to a power of
     // two (n2 <= 64), the 64 / NP parts share the inputs and a butterfly over
     // the part bits of the lane sums them
-    int NP = 1;
-    while (NP < n2) NP <<= 1;
-    const int node = lane & (NP - 1), part = lane / NP;
-    const int per = (n1 * NP + WAVE - 1) / WAVE;
+    // (shifts: NP is not known at compile time, and a division by it
+    // would be a full integer division)
+    int lg = 0;
+    while ((1 << lg) < n2) ++lg;
+    const int NP = 1 << lg;
+    const int node = lane & (NP - 1), part = lane >> lg;
+    const int per = (n1 * NP + WAVE - 1) >> 6;
     const int c0 = min(n1, part * per), c1 = min(n1, c0 + per);
     float z0 = 0.0f, z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;
     if (node < n2) {
@@ -272,10 +275,11 @@ __device__ inline float network(const ModelDev &md, const SpeciesDev &sp, WPtr W
     const WPtr Wl = W + sp.w_off[l];
     if (n1 <= WAVE / 2) {
       // lanes = (input c, part) as in the forward pass, parts share the nodes
-      int NP = 1;
-      while (NP < n1) NP <<= 1;
-      const int c = lane & (NP - 1), part = lane / NP;
-      const int per = (n2 * NP + WAVE - 1) / WAVE;
+      int lg = 0;
+      while ((1 << lg) < n1) ++lg;
+      const int NP = 1 << lg;
+      const int c = lane & (NP - 1), part = lane >> lg;
+      const int per = (n2 * NP + WAVE - 1) >> 6;
       const int h0 = min(n2, part * per), h1 = min(n2, h0 + per);
       float s0 = 0.0f, s1 = 0.0f;
       if (c < n1) {
