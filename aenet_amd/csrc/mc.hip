@@ -83,6 +83,25 @@ __device__ __forceinline__ double dist(const MCDev &m, const double *a, const do
   return sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
 }
 
+// the same in FP32 after the FP64 difference (the first stage's pair terms
+// are FP32: their distances need no more)
+__device__ __forceinline__ float dist_f(const MCDev &m, const double *a, const double *b) {
+  const float v[3] = {(float)(b[0] - a[0]), (float)(b[1] - a[1]), (float)(b[2] - a[2])};
+  float f[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    f[k] = (float)m.Binv[k] * v[0] + (float)m.Binv[3 + k] * v[1] + (float)m.Binv[6 + k] * v[2];
+    f[k] -= rintf(f[k]);
+  }
+  float d2 = 0.0f;
+#pragma unroll
+  for (int x = 0; x < 3; ++x) {
+    const float d = f[0] * (float)m.L[x] + f[1] * (float)m.L[3 + x] + f[2] * (float)m.L[6 + x];
+    d2 += d * d;
+  }
+  return __builtin_amdgcn_sqrtf(d2);
+}
+
 constexpr int SR = 16;  // radial terms of the first-stage descriptor
 constexpr int SW = 16;  // first-stage hidden width
 
@@ -134,8 +153,9 @@ template <int RN, class T>
 __device__ __forceinline__ void radial_terms_f(float Rc, int rN, double r, float w_plain, float w_spin, T *vr, T *vs) {
   const float rf = (float)r;
   if (!(rf <= Rc && rf > 1.0e-12f)) return;
-  const float fc = 0.5f * (__cosf(3.14159265f * rf / Rc) + 1.0f);
-  const float x = 2.0f * rf / Rc - 1.0f;
+  const float q = rf * __builtin_amdgcn_rcpf(Rc);
+  const float fc = 0.5f * (__cosf(3.14159265f * q) + 1.0f);
+  const float x = 2.0f * q - 1.0f;
   float Tm = 1.0f, Tc = x;
 #pragma unroll
   for (int n = 0; n < RN; ++n) {
@@ -388,15 +408,26 @@ __device__ __forceinline__ Move draw_move(const MCDev &m, const int *pt, uint64_
   return mv;
 }
 
-// the first-stage descriptor of atom j (species S) after the move mv:
-// positions sx and types pt of the chain with the move applied, xold the
-// displaced atom's old position, Gm the moved sites' descriptors, R the
-// chain's current descriptors; false (G untouched) when the move leaves
-// atom j's descriptor unchanged
+// A proposal evaluated from the chain's current state, the move not
+// applied: positions sx and types pt are the chain's, xnew the displaced
+// atom's new position.  type_after: atom j's species after the move
+// (0-based)
+__device__ __forceinline__ int type_after(const Move &mv, const int *pt, int j) {
+  if (mv.kind == MC_SWAP) {
+    if (j == mv.a) return pt[mv.b] - 1;
+    if (j == mv.b) return pt[mv.a] - 1;
+  }
+  return pt[j] - 1;
+}
+
+// the first-stage descriptor of atom j (species S after the move) under the
+// proposal mv: Gm the proposal's moved-site descriptors, R the chain's
+// current descriptors; false (G untouched) when the move leaves atom j's
+// descriptor unchanged
 template <int RN>
-__device__ __forceinline__ bool trial_desc(const MCDev &m, const FirstStage &S, int j, const Move &mv,
-                                           const double *sx, const int *pt, const double *xold,
-                                           const double (*Gm)[2 * SR], const FSRead &R, SDesc &G) {
+__device__ __forceinline__ bool trial_desc_view(const MCDev &m, const FirstStage &S, int j, const Move &mv,
+                                                const double *sx, const int *pt, const double *xnew,
+                                                const double (*Gm)[2 * SR], const FSRead &R, SDesc &G) {
   if (j == mv.a || (mv.kind == MC_SWAP && j == mv.b)) {
     const int q = j == mv.a ? 0 : 1;
 #pragma unroll
@@ -406,11 +437,11 @@ __device__ __forceinline__ bool trial_desc(const MCDev &m, const FirstStage &S, 
     }
     return true;
   }
-  const double Rc = S.Rc;
+  const float Rc = S.Rc;
   if (mv.kind == MC_DISP) {
-    // atom a left xold for its new position
-    const double ro = dist(m, &sx[3 * j], xold);
-    const double rn = dist(m, &sx[3 * j], &sx[3 * mv.a]);
+    // atom a leaves its position sx[a] for xnew
+    const float ro = dist_f(m, &sx[3 * j], &sx[3 * mv.a]);
+    const float rn = dist_f(m, &sx[3 * j], xnew);
     if (!(ro <= Rc || rn <= Rc)) return false;
     fs_load<RN>(R, S, j, G);
     const float spa = S.spin[pt[mv.a] - 1];
@@ -418,42 +449,41 @@ __device__ __forceinline__ bool trial_desc(const MCDev &m, const FirstStage &S, 
     radial_terms_f<RN>(S.Rc, S.rN, rn, 1.0f, spa, G.r, G.s);
     return true;
   }
-  // swap: a and b exchanged species; only the typespin weights of the
+  // swap: a and b exchange species; only the typespin weights of the
   // pairs (j, a) and (j, b) change
-  const double ra = dist(m, &sx[3 * j], &sx[3 * mv.a]);
-  const double rb = dist(m, &sx[3 * j], &sx[3 * mv.b]);
+  const float ra = dist_f(m, &sx[3 * j], &sx[3 * mv.a]);
+  const float rb = dist_f(m, &sx[3 * j], &sx[3 * mv.b]);
   if (!(ra <= Rc || rb <= Rc)) return false;
   fs_load<RN>(R, S, j, G);
-  const float ds = S.spin[pt[mv.a] - 1] - S.spin[pt[mv.b] - 1];  // new minus old weight of the pair (j, a)
+  const float ds = S.spin[pt[mv.b] - 1] - S.spin[pt[mv.a] - 1];  // new minus old weight of the pair (j, a)
   radial_terms_f<RN>(S.Rc, S.rN, ra, 0.0f, ds, G.r, G.s);
   radial_terms_f<RN>(S.Rc, S.rN, rb, 0.0f, -ds, G.r, G.s);
   return true;
 }
 
-// the moved sites' first-stage descriptors over the whole cell: per-lane
-// FP32 pair terms, a reduce-scatter over the wavefront (first level FP32,
-// then FP64; lane l ends with feature l/2), the waves' partials in
-// part[wave][q][f], summed in a fixed order into Gm by the caller
+// the first-stage descriptor of moved site q (0: a, 1: b) of proposal mv
+// over the whole cell, by one wavefront, the move not applied: per-lane
+// FP32 pair terms over the lane's atoms, a reduce-scatter over the
+// wavefront (first level FP32, then FP64; lane l ends with feature l/2),
+// written to G[2*SR] (plain terms, then typespin)
 template <int RN>
-__device__ __forceinline__ void moved_site_partials(const MCDev &m, const FirstStage *tab, const Move &mv,
-                                                    const double *sx, const int *pt, double *part) {
+__device__ __forceinline__ void moved_site_wave(const MCDev &m, const FirstStage *tab, const Move &mv, int q,
+                                                const double *sx, const int *pt, const double *xnew, double *G) {
   const int N = m.natoms;
-  const int lane = lane_id(), wave = threadIdx.x / WAVE;
-  for (int q = 0; q < 2; ++q) {
-    if (q == 1 && mv.kind != MC_SWAP) break;
-    const int st = q == 0 ? mv.a : mv.b;
-    const FirstStage &S = tab[pt[st] - 1];
-    float v[2 * SR];
+  const int lane = lane_id();
+  const int st = q == 0 ? mv.a : mv.b;
+  const FirstStage &S = tab[type_after(mv, pt, st)];
+  const double *xs = (mv.kind == MC_DISP && q == 0) ? xnew : &sx[3 * st];
+  float v[2 * SR];
 #pragma unroll
-    for (int f = 0; f < 2 * SR; ++f) v[f] = 0.0f;
-    for (int j = threadIdx.x; j < N; j += blockDim.x) {
-      if (j == st) continue;
-      const double r = dist(m, &sx[3 * st], &sx[3 * j]);
-      radial_terms_f<RN>(S.Rc, S.rN, r, 1.0f, S.spin[pt[j] - 1], v, v + SR);
-    }
-    const double t = reduce_scatter<2 * SR>(v);
-    if ((lane & 1) == 0) part[(wave * 2 + q) * 2 * SR + (lane >> 1)] = t;
+  for (int f = 0; f < 2 * SR; ++f) v[f] = 0.0f;
+  for (int j = lane; j < N; j += WAVE) {
+    if (j == st) continue;
+    const float r = dist_f(m, xs, &sx[3 * j]);
+    radial_terms_f<RN>(S.Rc, S.rN, r, 1.0f, S.spin[type_after(mv, pt, j)], v, v + SR);
   }
+  const double t = reduce_scatter<2 * SR>(v);
+  if ((lane & 1) == 0) G[lane >> 1] = t;
 }
 
 // per-chain bookkeeping held in registers by one thread while a kernel
@@ -535,12 +565,15 @@ __global__ void k_mc_propose(MCDev m) {
   m.rng[c] = s;
 }
 
-// LDS of k_mc_screen in doubles: positions [3N], trial energies [N], wave
-// partials [4][2][2*SR], the first-stage tables, with the cache the current
-// energies [N] and descriptors [N][2*rn], then types [N] and affected
-// flags [N] (ints)
+// proposals a screen round evaluates together (from the same chain state)
+constexpr int SCREEN_ROUND = 4;
+
+// LDS of k_mc_screen in doubles: positions [3N], trial energies of the
+// round's proposals [SCREEN_ROUND][N], the first-stage tables, with the
+// cache the current energies [N] and descriptors [N][2*rn], then types [N]
+// and the proposals that change each atom's descriptor [N] (ints, bit p)
 __host__ __device__ inline size_t screen_lds_bytes(int N, int ntypes, int rn, bool cache) {
-  size_t d = 3 * (size_t)N + N + 4 * 2 * 2 * SR;
+  size_t d = 3 * (size_t)N + SCREEN_ROUND * (size_t)N;
   if (cache) d += (size_t)N * (1 + 2 * rn);
   return 8 * d + ntypes * sizeof(FirstStage) + 2 * 4 * (size_t)N;
 }
@@ -549,29 +582,36 @@ __host__ __device__ inline size_t screen_lds_bytes(int N, int ntypes, int rn, bo
 // workgroup); the kernels read it with uniform addresses
 __global__ __launch_bounds__(256) void k_first_stage_table(ModelDev md, FirstStage *tab) { load_first_stage(md, tab); }
 
-// one workgroup per chain, threads over the chain's atoms; CACHE: the
+// One workgroup per chain, threads over the chain's atoms; CACHE: the
 // chain's first-stage state is copied into LDS once per step; RN, NH: the
-// first stage's radial terms and hidden nodes at most.  The network runs
-// once per species present among a wavefront's affected atoms, with the
+// first stage's radial terms and hidden nodes at most.  A rejection by the
+// first stage leaves the chain where it is, so every proposal of a step
+// starts from the same state: the proposals are drawn SCREEN_ROUND at a
+// time, evaluated together (each thread over its atoms for every proposal),
+// then decided in the order drawn; the first that passes goes on to the
+// full potential and the ones after it are dropped (their random numbers
+// unused), which is sequential delayed acceptance.  The network runs once
+// per species present among a wavefront's affected atoms, with the
 // weights of tabg (uniform loads); the LDS copy of the table serves the
 // per-lane lookups
 template <bool CACHE, int RN, int NH>
 __global__ __launch_bounds__(256) void k_mc_screen(MCDev m, ModelDev md, const FirstStage *__restrict__ tabg) {
   extern __shared__ double lds[];
-  __shared__ double Gm[2][2 * SR];  // descriptors of the moved sites
-  __shared__ double red[4];
-  __shared__ Move smv;
-  __shared__ double xold[3];
-  __shared__ int s_pass;
+  __shared__ double Gm[SCREEN_ROUND][2][2 * SR];  // descriptors of each proposal's moved sites
+  __shared__ double red[4][SCREEN_ROUND];
+  __shared__ Move smv[SCREEN_ROUND];
+  __shared__ double xnew[SCREEN_ROUND][3];
+  __shared__ int s_live[SCREEN_ROUND];  // proposal p is part of this step (k < max_screen)
+  __shared__ int s_pick;
   const int c = blockIdx.x;
   const int N = m.natoms;
   const int rn = m.rn_s;
   const size_t base = (size_t)c * N;
   const int nwaves = (blockDim.x + WAVE - 1) / WAVE;
+  const int wave = threadIdx.x / WAVE;
   double *sx = lds;
-  double *sE = sx + 3 * N;
-  double *part = sE + N;
-  FirstStage *tab = reinterpret_cast<FirstStage *>(part + 4 * 2 * 2 * SR);
+  double *sE = sx + 3 * N;  // [SCREEN_ROUND][N]
+  FirstStage *tab = reinterpret_cast<FirstStage *>(sE + SCREEN_ROUND * N);
   double *Ec = reinterpret_cast<double *>(tab + md.ntypes);
   double *Gc = Ec + (CACHE ? N : 0);
   int *pt = reinterpret_cast<int *>(Gc + (CACHE ? (size_t)N * 2 * rn : 0));
@@ -585,11 +625,21 @@ __global__ __launch_bounds__(256) void k_mc_screen(MCDev m, ModelDev md, const F
   for (int k = threadIdx.x; k < N; k += blockDim.x) pt[k] = m.types[base + k];
   __syncthreads();
   if (CACHE) {
-    for (int k = threadIdx.x; k < N * 2 * rn; k += blockDim.x) {
-      const int j = k / (2 * rn), f = k - j * 2 * rn;
+    // k = j * 2rn + f stepped by the workgroup size: (j, f) advanced by
+    // (bs / 2rn, bs mod 2rn) with a carry instead of a division per element
+    const int w2 = 2 * rn, bs = blockDim.x;
+    const int dj = bs / w2, df = bs - dj * w2;
+    int j = threadIdx.x / w2, f = threadIdx.x - j * w2;
+    for (int k = threadIdx.x; k < N * w2; k += bs) {
       const FirstStage &S = tab[pt[j] - 1];
       const int q = f >= rn, n = f - q * rn;
       Gc[k] = (n < S.rN && (q == 0 || S.multi)) ? m.G_s[(base + j) * m.nsf_s + (q ? S.r_i2 : S.r_i1) + n] : 0.0;
+      j += dj;
+      f += df;
+      if (f >= w2) {
+        f -= w2;
+        ++j;
+      }
     }
     for (int k = threadIdx.x; k < N; k += blockDim.x) Ec[k] = m.E_s[base + k];
   }
@@ -602,105 +652,117 @@ __global__ __launch_bounds__(256) void k_mc_screen(MCDev m, ModelDev md, const F
     L = log_open(m, c);
   }
   double dE_pass = 0.0;  // thread 0: first-stage change of the proposal that passed
-  bool pass = false;
-  for (int t = 0; t < m.max_screen && !pass; ++t) {
-    __syncthreads();  // the previous try's shared values are no longer read
+  int pick = -1;
+  for (int k0 = 0; k0 < m.max_screen && pick < 0; k0 += SCREEN_ROUND) {
+    __syncthreads();  // the previous round's shared values are no longer read
     if (threadIdx.x == 0) {
-      const Move mv = draw_move(m, pt, s);
-      if (mv.kind == MC_SWAP) {
-        const int ta = pt[mv.a];
-        pt[mv.a] = pt[mv.b];
-        pt[mv.b] = ta;
-      } else if (mv.kind == MC_DISP) {
-        for (int x = 0; x < 3; ++x) {
-          xold[x] = sx[3 * mv.a + x];
-          sx[3 * mv.a + x] = xold[x] + mv.d[x];
+      for (int p = 0; p < SCREEN_ROUND; ++p) {
+        const bool live = k0 + p < m.max_screen;
+        Move mv;
+        if (live) {
+          mv = draw_move(m, pt, s);
+        } else {
+          mv.kind = MC_NONE;
+          mv.a = mv.b = 0;
         }
+        if (mv.kind == MC_DISP)
+          for (int x = 0; x < 3; ++x) xnew[p][x] = sx[3 * mv.a + x] + mv.d[x];
+        smv[p] = mv;
+        s_live[p] = live ? 1 : 0;
       }
-      smv = mv;
     }
     __syncthreads();
-    const Move mv = smv;
-    if (mv.kind == MC_NONE) continue;  // no atom of another species: not a proposal
-    moved_site_partials<RN>(m, tab, mv, sx, pt, part);
-    __syncthreads();
-    if (threadIdx.x < 4 * SR) {
-      const int q = threadIdx.x / (2 * SR), f = threadIdx.x - q * 2 * SR;
-      double g = 0.0;
-      for (int w = 0; w < nwaves; ++w) g += part[(w * 2 + q) * 2 * SR + f];
-      Gm[q][f] = g;
+    // the moved sites' descriptors of every proposal of the round, a
+    // wavefront each in turn
+    {
+      int i = 0;
+      for (int p = 0; p < SCREEN_ROUND; ++p) {
+        const Move mv = smv[p];
+        if (mv.kind == MC_NONE) continue;  // not a proposal (uniform)
+        const int nq = mv.kind == MC_SWAP ? 2 : 1;
+        for (int q = 0; q < nq; ++q, ++i)
+          if (i % nwaves == wave) moved_site_wave<RN>(m, tab, mv, q, sx, pt, xnew[p], Gm[p][q]);
+      }
     }
     __syncthreads();
-    double dE = 0.0;
-    // uniform loops, so that the species index and the table address are
-    // wave-uniform (scalar loads of the weights)
+    // every proposal's change over the affected atoms; uniform loops, so
+    // that the species index and the table address are wave-uniform
+    // (scalar loads of the weights)
     for (int j0 = 0; j0 < N; j0 += blockDim.x) {
       const int j = j0 + threadIdx.x;
       const bool on = j < N;
-      const int tj = on ? pt[j] - 1 : 0;
-      SDesc G;
-      const bool aff = on && trial_desc<RN>(m, tab[tj], j, mv, sx, pt, xold, Gm, R, G);
-      double E = 0.0;
-      for (int t = 0; t < md.ntypes; ++t) {
-        const bool mine = aff && tj == t;
-        if (wave_ballot(mine)) {
-          // the species' table through the constant address space: scalar loads
-          const ConstFirstStage *St = (const ConstFirstStage *)(tabg + uniform(t));
-          if (mine) E = fs_energy<RN, NH>(*St, G);
+      int bits = 0;
+      for (int p = 0; p < SCREEN_ROUND; ++p) {
+        const Move mv = smv[p];
+        double dE = 0.0;
+        if (mv.kind != MC_NONE) {
+          const int tj = on ? type_after(mv, pt, j) : 0;
+          SDesc G;
+          const bool aff = on && trial_desc_view<RN>(m, tab[tj], j, mv, sx, pt, xnew[p], Gm[p], R, G);
+          double E = 0.0;
+          for (int t = 0; t < md.ntypes; ++t) {
+            const bool mine = aff && tj == t;
+            if (wave_ballot(mine)) {
+              // the species' table through the constant address space: scalar loads
+              const ConstFirstStage *St = (const ConstFirstStage *)(tabg + uniform(t));
+              if (mine) E = fs_energy<RN, NH>(*St, G);
+            }
+          }
+          if (aff) {
+            sE[p * N + j] = E;
+            dE = E - Ecur[j];
+            bits |= 1 << p;
+          }
         }
+        dE = wave_sum(dE);
+        if (lane_id() == 0) red[wave][p] = j0 == 0 ? dE : red[wave][p] + dE;
       }
-      if (aff) {
-        sE[j] = E;
-        dE += E - Ecur[j];
-      }
-      if (on) saff[j] = aff ? 1 : 0;
+      if (on) saff[j] = bits;
     }
-    dE = wave_sum(dE);
-    if (lane_id() == 0) red[threadIdx.x / WAVE] = dE;
     __syncthreads();
     if (threadIdx.x == 0) {
-      double tot = 0.0;
-      for (int w = 0; w < nwaves; ++w) tot += red[w];
-      const bool p = log(mv.u1) < -m.beta * tot;
-      s_pass = p ? 1 : 0;
-      dE_pass = tot;
-      if (!p) {
-        // rejected by the first stage: the chain stays where it is
-        log_proposal(m, c, L, mv.kind, tot, 0.0, 0, false);
-        if (mv.kind == MC_SWAP) {
-          const int ta = pt[mv.a];
-          pt[mv.a] = pt[mv.b];
-          pt[mv.b] = ta;
+      // decisions in the order drawn: the first that passes goes on
+      for (int p = 0; p < SCREEN_ROUND && pick < 0; ++p) {
+        const Move mv = smv[p];
+        if (!s_live[p] || mv.kind == MC_NONE) continue;  // no atom of another species: not a proposal
+        double tot = 0.0;
+        for (int w = 0; w < nwaves; ++w) tot += red[w][p];
+        if (log(mv.u1) < -m.beta * tot) {
+          pick = p;
+          dE_pass = tot;
         } else {
-          for (int x = 0; x < 3; ++x) sx[3 * mv.a + x] = xold[x];
+          // rejected by the first stage: the chain stays where it is
+          log_proposal(m, c, L, mv.kind, tot, 0.0, 0, false);
         }
       }
+      s_pick = pick;
     }
     __syncthreads();
-    pass = s_pass != 0;
+    pick = s_pick;
   }
-  if (pass) {
+  if (pick >= 0) {
     // the proposal that passed goes to the full potential, applied in place
-    const Move mv = smv;
+    const Move mv = smv[pick];
     for (int j = threadIdx.x; j < N; j += blockDim.x) {
       const size_t g = base + j;
-      if (saff[j]) {
-        const FirstStage &S = tab[pt[j] - 1];
+      const bool aff = (saff[j] >> pick) & 1;
+      if (aff) {
+        const FirstStage &S = tab[type_after(mv, pt, j)];
         SDesc G;
-        trial_desc<RN>(m, S, j, mv, sx, pt, xold, Gm, R, G);
+        trial_desc_view<RN>(m, S, j, mv, sx, pt, xnew[pick], Gm[pick], R, G);
         fs_store(S, m.nsf_s, m.G_trial + g * m.nsf_s, G);
-        m.E_s_trial[g] = sE[j];
+        m.E_s_trial[g] = sE[pick * N + j];
       }
-      m.affected[g] = saff[j];
+      m.affected[g] = aff ? 1 : 0;
     }
     if (threadIdx.x == 0) {
       if (mv.kind == MC_SWAP) {
-        m.types[base + mv.a] = pt[mv.a];
-        m.types[base + mv.b] = pt[mv.b];
+        m.types[base + mv.a] = pt[mv.b];
+        m.types[base + mv.b] = pt[mv.a];
       } else {
         for (int x = 0; x < 3; ++x) {
-          m.old_pos[3 * c + x] = xold[x];
-          m.coords[3 * (base + mv.a) + x] = sx[3 * mv.a + x];
+          m.old_pos[3 * c + x] = sx[3 * mv.a + x];
+          m.coords[3 * (base + mv.a) + x] = xnew[pick][x];
         }
       }
       m.kind[c] = mv.kind;
@@ -765,43 +827,85 @@ __global__ __launch_bounds__(1024) void k_mc_active(MCDev m) {
 }
 
 // one wavefront per chain
-__global__ __launch_bounds__(64) void k_mc_finish(MCDev m, int plain) {
+// one workgroup of up to 4 wavefronts per chain: dE of the full potential
+// over its block, the decision, then commit or revert; the affected atoms'
+// first-stage state is copied by the wavefronts in parallel, a chunk of 64
+// atoms each
+__global__ __launch_bounds__(256) void k_mc_finish(MCDev m, int plain) {
+  __shared__ double red[4];
   const int c = blockIdx.x;
-  const int lane = threadIdx.x;
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid / WAVE;
+  const int nw = (blockDim.x + WAVE - 1) / WAVE;
   const int N = m.natoms;
   const size_t base = (size_t)c * N;
   const int kind = m.kind[c];
   const bool need = m.need[c] != 0;
+  const int s0 = m.act_start[c], n = need ? m.act_count[c] : 0;
   double dE = 0.0;
-  if (need) {
-    const int s0 = m.act_start[c], n = m.act_count[c];
-    for (int k = lane; k < n; k += WAVE) {
-      const int g = m.active[s0 + k];
-      dE += m.E_trial[g] - m.E_t[g];
-    }
+  for (int k = tid; k < n; k += blockDim.x) {
+    const int g = m.active[s0 + k];
+    dE += m.E_trial[g] - m.E_t[g];
   }
   dE = wave_sum(dE);
+  if (lane == 0) red[wave] = dE;
+  __syncthreads();
+  dE = 0.0;
+  for (int w = 0; w < nw; ++w) dE += red[w];  // fixed order
   bool accept = false;
   if (need) {
     const double arg = plain ? -m.beta * dE : -m.beta * (dE - m.dE_s[c]);
     accept = log(m.u_stage2[c]) < arg;
   }
   if (accept) {
-    const int s0 = m.act_start[c], n = m.act_count[c];
-    for (int k = lane; k < n; k += WAVE) {
+    for (int k = tid; k < n; k += blockDim.x) {
       const int g = m.active[s0 + k];
       m.E_t[g] = m.E_trial[g];
     }
     if (!plain) {
-      for (int j = lane; j < N; j += WAVE) {
-        const size_t g = base + j;
-        if (m.affected[g]) {
-          for (int f = 0; f < m.nsf_s; ++f) m.G_s[g * m.nsf_s + f] = m.G_trial[g * m.nsf_s + f];
-          m.E_s[g] = m.E_s_trial[g];
+      // the affected atoms' first-stage state: their rows copied by groups
+      // of nsf consecutive lanes (rows = 64 / nsf atoms a pass), the loads
+      // of four passes issued before their stores
+      const int nsf = m.nsf_s;
+      const double *__restrict__ src = m.G_trial;
+      double *__restrict__ dst = m.G_s;
+      const int rows = nsf <= WAVE ? WAVE / nsf : 1;
+      const int r = nsf <= WAVE ? lane / nsf : 0;
+      for (int j0 = wave * WAVE; j0 < N; j0 += nw * WAVE) {
+        const int j = j0 + lane;
+        const bool aff = j < N && m.affected[base + j] != 0;
+        if (aff) m.E_s[base + j] = m.E_s_trial[base + j];
+        unsigned long long msk = wave_ballot(aff);
+        while (msk) {
+          size_t at[4];
+          double val[4];
+          bool ok[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            // this lane's row of the pass: the r-th set bit left in msk
+            unsigned long long mm = msk;
+            for (int k = 0; k < r && mm; ++k) mm &= mm - 1;
+            ok[u] = mm != 0 && r < rows;
+            at[u] = ok[u] ? (base + j0 + __builtin_ctzll(mm)) * nsf : 0;
+            for (int k = 0; k < rows && msk; ++k) msk &= msk - 1;
+          }
+          if (nsf <= WAVE) {
+            const int f = lane - r * nsf;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (ok[u]) val[u] = src[at[u] + f];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (ok[u]) dst[at[u] + f] = val[u];
+          } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (ok[u])
+                for (int f = lane; f < nsf; f += WAVE) dst[at[u] + f] = src[at[u] + f];
+          }
         }
       }
     }
-  } else if (lane == 0 && need) {
+  } else if (tid == 0 && need) {
     const int a = m.site_a[c], b = m.site_b[c];
     if (kind == MC_SWAP) {
       const int ta = m.types[base + a];
@@ -811,7 +915,7 @@ __global__ __launch_bounds__(64) void k_mc_finish(MCDev m, int plain) {
       for (int x = 0; x < 3; ++x) m.coords[3 * (base + a) + x] = m.old_pos[3 * c + x];
     }
   }
-  if (lane == 0) {
+  if (tid == 0) {
     if (accept) m.E_chain[c] += dE;
     if (need) {
       ChainLog L = log_open(m, c);
@@ -1273,7 +1377,7 @@ void launch_mc_step(const MCDev &m, const ModelDev *student, bool plain, int pha
     }
     hipLaunchKernelGGL(k_mc_active, dim3(C), dim3(bs), 0, s, m);
   } else {
-    hipLaunchKernelGGL(k_mc_finish, dim3(C), dim3(64), 0, s, m, plain ? 1 : 0);
+    hipLaunchKernelGGL(k_mc_finish, dim3(C), dim3(std::min(256, bs)), 0, s, m, plain ? 1 : 0);
   }
 }
 
