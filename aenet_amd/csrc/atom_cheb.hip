@@ -598,21 +598,26 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
         float x = 2.0f * r * inv_rRc - 1.0f;
         float sj = multi ? sp.spin[S.t[q] - 1] : 0.0f;
         float fcx = fc * 2.0f * inv_rRc;
-        // sigma = sum_n a_n (dfc T_n + fc T'_n dx/dr), T'_n = n U_{n-1}
-        float sig = rcoef(0, sj) * dfc;
-        float Tm = 1.0f, T = x, Um = 0.0f, U = 1.0f;
+        // sigma = dfc S + fc S' dx/dr with S = sum_n a_n T_n(x), both sums by
+        // Clenshaw's recurrence: b_n = a_n + 2x b_n+1 - b_n+2 gives
+        // S = a_0 + x b_1 - b_2, and its derivative d_n = 2 b_n+1 + 2x d_n+1 -
+        // d_n+2 gives S' = b_1 + x d_1 - d_2 (zero coefficients beyond the
+        // species' rN)
+        const float x2 = 2.0f * x;
+        float b1 = 0.0f, b2 = 0.0f, d1 = 0.0f, d2 = 0.0f;
 #pragma unroll
-        for (int k = 1; k < RN; ++k) {
-          // zero coefficients beyond the species' rN
+        for (int k = RN - 1; k >= 1; --k) {
           const float a = rcoef(k, sj);
-          sig += a * (dfc * T + fcx * (float)k * U);
-          float Tn = 2.0f * x * T - Tm;
-          float Un = 2.0f * x * U - Um;
-          Tm = T;
-          T = Tn;
-          Um = U;
-          U = Un;
+          const float b0 = fmaf(x2, b1, a - b2);
+          const float d0 = fmaf(x2, d1, fmaf(2.0f, b1, -d2));
+          b2 = b1;
+          b1 = b0;
+          d2 = d1;
+          d1 = d0;
         }
+        const float Sv = fmaf(x, b1, rcoef(0, sj) - b2);
+        const float Sd = fmaf(x, d1, b1 - d2);
+        const float sig = fmaf(dfc, Sv, fcx * Sd);
         float s = sig / r;
         gx = s * S.R[3 * q];
         gy = s * S.R[3 * q + 1];
@@ -642,24 +647,22 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
       const float fcj = Aj.y, dfcj = Aj.z, fck = Ak.y, dfck = Ak.z;
       float w = fcj * fck;
       float ss = Aj.w * Ak.w;
-      // A = sum_n (p_n + ss s_n) T_n(cos), B = sum_n (p_n + ss s_n) T'_n(cos)
-      float Ap = cap_[0], As = cas_[0], Bp = 0.0f, Bs = 0.0f;
-      float Tm = 1.0f, T = cosv, Um = 0.0f, U = 1.0f;
+      // A = sum_n c_n T_n(cos), B = sum_n c_n T'_n(cos), c_n = p_n + ss s_n,
+      // by Clenshaw's recurrence and its derivative (as in the radial pass)
+      const float c2 = 2.0f * cosv;
+      float b1 = 0.0f, b2 = 0.0f, d1 = 0.0f, d2 = 0.0f;
 #pragma unroll
-      for (int k = 1; k < AN; ++k) {
-        Ap += cap_[k] * T;
-        As += cas_[k] * T;
-        Bp += cap_[k] * (float)k * U;
-        Bs += cas_[k] * (float)k * U;
-        float Tn = 2.0f * cosv * T - Tm;
-        float Un = 2.0f * cosv * U - Um;
-        Tm = T;
-        T = Tn;
-        Um = U;
-        U = Un;
+      for (int k = AN - 1; k >= 1; --k) {
+        const float c = fmaf(ss, cas_[k], cap_[k]);
+        const float b0 = fmaf(c2, b1, c - b2);
+        const float d0 = fmaf(c2, d1, fmaf(2.0f, b1, -d2));
+        b2 = b1;
+        b1 = b0;
+        d2 = d1;
+        d1 = d0;
       }
-      const float A = Ap + ss * As;
-      const float Bd = Bp + ss * Bs;
+      const float A = fmaf(cosv, b1, fmaf(ss, cas_[0], cap_[0]) - b2);
+      const float Bd = fmaf(cosv, d1, b1 - d2);
       float wB = w * Bd;
       float aj = A * dfcj * fck * ij, ak = A * fcj * dfck * ik;
       float cj = cosv * ij * ij, ck = cosv * ik * ik;
