@@ -249,6 +249,73 @@ __global__ __launch_bounds__(1024) void k_bin_small(BatchDev b) {
   }
 }
 
+// The binning of a small batch after k_frac with one workgroup per cell
+// instead of one for the batch: each workgroup reads every atom's cell,
+// counts the atoms of lower cells (its cell's start in cell order) and
+// places its own atoms in descending index order (the reference's linked
+// lists), with their lattice coordinates and the cell's search work items.
+// Wavefront w takes the contiguous atom range [N - (w+1) P, N - w P) in
+// chunks of 64 going down, lane l the atom top - 64 u - l of chunk u
+// (coalesced loads, all issued first); ballots rank the cell's atoms within
+// a chunk and running counts across chunks and wavefronts.  The results are
+// those of k_bin_small.
+constexpr int BIN_CELLS_THREADS = 256, BIN_CELLS_WAVES = BIN_CELLS_THREADS / WAVE;
+constexpr int BIN_CELLS_CHUNKS = SMALL_ATOMS / BIN_CELLS_THREADS;  // chunks of 64 per wavefront at most
+constexpr long BIN_CELLS_WORK = 4L << 20;                          // natoms x ncells
+__global__ __launch_bounds__(BIN_CELLS_THREADS) void k_bin_cells(BatchDev b) {
+  __shared__ int wb[BIN_CELLS_WAVES], wm[BIN_CELLS_WAVES];
+  const int c = blockIdx.x;
+  const int tid = threadIdx.x, w = tid / WAVE, lane = lane_id();
+  const int N = b.natoms;
+  const int P = (N + BIN_CELLS_THREADS - 1) / BIN_CELLS_THREADS * WAVE;  // atoms per wavefront, multiple of 64
+  const int top = N - w * P, bottom = max(top - P, 0);
+  int cc[BIN_CELLS_CHUNKS];
+#pragma unroll
+  for (int u = 0; u < BIN_CELLS_CHUNKS; ++u) {
+    const int g = top - 1 - WAVE * u - lane;
+    cc[u] = g >= bottom ? b.cellof[g] : -1;
+  }
+  int before = 0, mine = 0;
+  unsigned long long mk[BIN_CELLS_CHUNKS];
+#pragma unroll
+  for (int u = 0; u < BIN_CELLS_CHUNKS; ++u) {
+    before += cc[u] >= 0 && cc[u] < c;
+    mk[u] = wave_ballot(cc[u] == c);
+    mine += popc64(mk[u]);
+  }
+  before = (int)wave_sum((long long)before);
+  if (lane == 0) {
+    wb[w] = before;
+    wm[w] = mine;
+  }
+  __syncthreads();
+  int start = 0, total = 0, off = 0;
+#pragma unroll
+  for (int v = 0; v < BIN_CELLS_WAVES; ++v) {
+    start += wb[v];
+    total += wm[v];
+    if (v < w) off += wm[v];
+  }
+  int q = start + off;
+#pragma unroll
+  for (int u = 0; u < BIN_CELLS_CHUNKS; ++u) {
+    if (cc[u] == c) {
+      const int g = top - 1 - WAVE * u - lane;
+      const int p = q + mask_prefix(mk[u]);
+      b.cell_atoms[p] = g;
+      b.sorted_s[3 * (size_t)p] = b.s[3 * (size_t)g];
+      b.sorted_s[3 * (size_t)p + 1] = b.s[3 * (size_t)g + 1];
+      b.sorted_s[3 * (size_t)p + 2] = b.s[3 * (size_t)g + 2];
+    }
+    q += popc64(mk[u]);
+  }
+  if (tid == 0) {
+    b.cell_start[c] = start;
+    if (c == b.ncells - 1) b.cell_start[b.ncells] = start + total;
+  }
+  write_cell_work(b, c, start, start + total, tid, BIN_CELLS_THREADS);
+}
+
 // exclusive scan of the cell counts (one workgroup)
 __global__ __launch_bounds__(1024) void k_scan_cells(BatchDev b) {
   __shared__ int wsum[16];
@@ -934,7 +1001,12 @@ void launch_bucket_atoms(const int *nb_count, int natoms, const BucketCaps &bc, 
 void launch_binning(const BatchDev &b, int *tmp, double *F, hipStream_t s) {
   if (b.natoms <= SMALL_ATOMS && b.ncells <= SMALL_CELLS) {
     hipLaunchKernelGGL(k_frac, dim3(nblk(b.natoms, 256)), dim3(256), 0, s, b, F);
-    hipLaunchKernelGGL(k_bin_small, dim3(1), dim3(1024), 0, s, b);
+    // a workgroup per cell while every workgroup's pass over all atoms stays
+    // small; one workgroup for the batch otherwise
+    if ((long)b.natoms * b.ncells <= BIN_CELLS_WORK)
+      hipLaunchKernelGGL(k_bin_cells, dim3(b.ncells), dim3(BIN_CELLS_THREADS), 0, s, b);
+    else
+      hipLaunchKernelGGL(k_bin_small, dim3(1), dim3(1024), 0, s, b);
     return;
   }
   // cell_count is zero on entry (cleared at allocation and by k_cell_sort)
