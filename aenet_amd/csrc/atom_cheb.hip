@@ -892,8 +892,10 @@ void launch_atom_cheb(int rN, int aN, int mode, bool forces, const AtomSizes &z,
     launch_cheb<17, 5>(mode, forces, md, b, nb, o, e, P, stream);
   else if (rN <= 19 && aN <= 7)
     launch_cheb<19, 7>(mode, forces, md, b, nb, o, e, P, stream);
-  else
+  else if (rN <= 24 && aN <= 8)
     launch_cheb<24, 8>(mode, forces, md, b, nb, o, e, P, stream);
+  else  // rN, aN <= 16 (cheb_orders_ok)
+    launch_cheb<16, 16>(mode, forces, md, b, nb, o, e, P, stream);
 }
 
 }  // namespace aenet_amd

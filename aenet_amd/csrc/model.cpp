@@ -284,8 +284,13 @@ int PotSet::finalize(std::string &err) {
       for (int g = 1; g <= ntypes; ++g) sd.spin[g - 1] = (float)typespin[p.stp.ltype[g]];
       max_rN = std::max(max_rN, sd.r_N);
       max_aN = std::max(max_aN, sd.a_N);
-      if (sd.r_N > 24 || sd.a_N > 8) {
-        err = p.file + ": Chebyshev orders above 23 (radial) / 7 (angular) are not supported";
+      if (!cheb_orders_ok(sd.r_N, sd.a_N)) {
+        err = p.file + ": Chebyshev orders beyond radial 23 with angular 7, or both 15, are not supported";
+        return AENET_STATUS_ERR_IO;
+      }
+      if (!cheb_orders_ok(max_rN, max_aN)) {
+        err = p.file + ": the set's Chebyshev orders together (radial " + std::to_string(max_rN - 1) + ", angular " +
+              std::to_string(max_aN - 1) + ") exceed radial 23 with angular 7, or both 15";
         return AENET_STATUS_ERR_IO;
       }
     } else {

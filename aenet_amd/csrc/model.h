@@ -38,6 +38,14 @@ struct PotHost {
   std::vector<double> ts_E_atom;
 };
 
+// Chebyshev term counts (order + 1) the atom kernel has a register layout
+// for: radial up to 24 with angular up to 8, or both up to 16 (the
+// per-lane accumulators of a descriptor hold 2 (radial + angular) <= 64
+// terms, typespin included)
+inline bool cheb_orders_ok(int radial_terms, int angular_terms) {
+  return (radial_terms <= 24 && angular_terms <= 8) || (radial_terms <= 16 && angular_terms <= 16);
+}
+
 // Reads one .nn file (Fortran unformatted sequential; 36 records).
 // Returns 0 on success, AENET_ERR_IO / a format error code otherwise.
 int read_nn(const std::string &path, PotHost &pot, std::string &err);

@@ -166,11 +166,23 @@ def mixed_set():
     return write_set("mixed", types, pots)
 
 
+def chebhi_set():
+    """A/B with typespin, radial order 10 / 6.0 A and angular order 12 /
+    4.0 A (48 features), 48-16-16-1: angular orders above 7 take the atom
+    kernel's (16, 16) register layout."""
+    cal = [synth.binary_fcc(3.61, (3, 3, 3), 0.1, 17, 18)]
+    return cheb_set("chebhi", ["A", "B"], [-1000.0, -1500.0], 6.0, 10, 4.0, 12, (16, 16), 70, cal)
+
+
 def main():
     os.makedirs(OUT, exist_ok=True)
     if sys.argv[1:] == ["mixed"]:
         mixed_set()
         print("wrote the mixed set to", OUT)
+        return
+    if sys.argv[1:] == ["chebhi"]:
+        chebhi_set()
+        print("wrote the chebhi set to", OUT)
         return
     # fcc108: single species, Chebyshev radial 16 / 6.0 A, angular 4 / 4.0 A, 22-20-20-1
     c, p = synth.fcc(3.61, (3, 3, 3))
@@ -200,6 +212,8 @@ def main():
     write_set("bp3", types, pots)
     # Behler-Parrinello with every function kind and mixed cutoffs, 2 species
     mixed_set()
+    # Chebyshev with an angular order above the radial one
+    chebhi_set()
     print("wrote potentials to", OUT)
 
 

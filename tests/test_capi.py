@@ -172,3 +172,26 @@ def test_bucket_caps():
     assert caps(list(range(1, 513)) * 40, 512) == [64, 128, 256, 512]
     assert caps(list(range(1, 513)) * 200, 512) == [16, 32, 64, 128, 256, 512]
     assert caps([], 64) == [64]
+
+
+def test_chebyshev_orders_beyond_the_register_layouts_are_refused(tmp_path):
+    """The Chebyshev atom kernel holds a descriptor's terms in per-lane
+    registers: radial order up to 23 with angular order up to 7, or both up
+    to 15 (the reference sizes them at run time).  A potential beyond both
+    layouts is refused when the set is complete (AENET_ERR_IO), before any
+    device work."""
+    from golden.make_potentials import make_pot
+    from aenet_amd import nnfile
+    L = _native.lib()
+    st = C.c_int(-1)
+    setup = nnfile.chebyshev_setup("A", ["A"], 6.0, 20, 4.0, 10)
+    pot = make_pot("A", setup, [setup["nsf"], 8, 1], 5, ["A"], [-1000.0])
+    path = str(tmp_path / "A.nn")
+    nnfile.write_nn(path, pot)
+    L.aenet_init(1, (C.c_char_p * 1)(b"A"), C.byref(st))
+    assert st.value == 0
+    try:
+        L.aenet_load_potential(1, path.encode(), C.byref(st))
+        assert st.value == _native.AENET_ERR_IO
+    finally:
+        L.aenet_final(C.byref(st))

@@ -35,6 +35,8 @@ def _structures():
     # Behler-Parrinello G1-G5, G2 with Rs = 0.5, radii 4.5 / 5.0 / 5.5 / 6.0 A
     for s in mixed_structures().values():
         out.append(("mixed", ["A", "B"], s))
+    # Chebyshev radial order 10, angular order 12
+    out.append(("chebhi", ["A", "B"], synth.binary_fcc(3.61, (3, 3, 3), 0.1, 19, 20)))
     return out
 
 
