@@ -6,7 +6,7 @@
 
 #define AENET_AMD_MAX_TYPES 8
 #define AENET_AMD_MAX_LAYERS 8
-#define AENET_AMD_MAX_HIDDEN 64      // hidden layer width handled by one wavefront
+#define AENET_AMD_MAX_HIDDEN 512     // hidden layer width (wider than a wavefront: in chunks of 64 nodes)
 #define AENET_AMD_MAX_NSF 256        // descriptor length per species
 #define AENET_AMD_NB_CAP 512         // neighbors held in LDS per atom
 #define AENET_AMD_KEPT_CAP 512       // (cell, image) combos kept per atom

@@ -553,7 +553,7 @@ struct AngForceFastOp {
 // MODE 0: structures (compact lists, fixed-point atomic force scatter)
 // MODE 1: explicit environments (per-neighbor forces out)
 // Registers are held to what BP_SIMD_WAVES wavefronts per SIMD allow.
-template <int MODE, bool FORCES>
+template <int MODE, bool FORCES, bool WIDE>
 __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) __attribute__((amdgpu_waves_per_eu(BP_SIMD_WAVES))) void k_atom_bp(
     ModelDev md, BatchDev b, NbDev nbl, OutDev o, EnvDev env, BPLaunch P) {
   extern __shared__ float lds[];
@@ -796,8 +796,8 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) __attribute__((amdgpu_waves_pe
       wave_sync();
       yout = 0.0f;
     } else {
-      yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES)
-                        : network(md, sp, (global_weights)md.W, S.net, FORCES);
+      yout = P.w_in_lds ? network<WIDE>(md, sp, (lds_weights)lds, S.net, FORCES)
+                        : network<WIDE>(md, sp, (global_weights)md.W, S.net, FORCES);
     }
     if (lane == 0) {
       if (MODE == 0)
@@ -940,7 +940,12 @@ static void launch_bp(const ModelDev &md, const BatchDev &b, const NbDev &nb, co
   }
   P.w_in_lds = ps.w_in_lds;
   P.waves = ps.waves;
-  hipLaunchKernelGGL((k_atom_bp<MODE, F>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream, md, b, nb, o, e, P);
+  if (P.hs > 64)  // hidden layers wider than the wavefront
+    hipLaunchKernelGGL((k_atom_bp<MODE, F, true>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream, md, b, nb, o, e,
+                       P);
+  else
+    hipLaunchKernelGGL((k_atom_bp<MODE, F, false>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream, md, b, nb, o, e,
+                       P);
 }
 
 static BPLaunch bp_launch(int mode, const AtomSizes &z) {

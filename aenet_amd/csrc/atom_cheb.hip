@@ -253,7 +253,7 @@ __device__ inline void network_rows(WPtr W, float *regions, const ChebLaunch &P,
 // MODE 0: structures (compact lists, fixed-point atomic force scatter)
 // MODE 1: explicit environments (per-neighbor forces out)
 // TILE: the network of a round's atoms as one matrix tile (MODE 0)
-template <int RN, int AN, int MODE, bool FORCES, bool TILE>
+template <int RN, int AN, int MODE, bool FORCES, bool TILE, bool WIDE>
 __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev md, BatchDev b, NbDev nbl, OutDev o, EnvDev env,
                                                           ChebLaunch P) {
   static_assert(2 * (RN + AN) <= 64, "features of one chunk");
@@ -536,8 +536,8 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
           else
             first_layer_from_lanes(sp, (global_weights)md.W, lane_f, lane_x, S.net);
         }
-        yout = P.w_in_lds ? network(md, sp, (lds_weights)lds, S.net, FORCES, true, first)
-                          : network(md, sp, (global_weights)md.W, S.net, FORCES, true, first);
+        yout = P.w_in_lds ? network<WIDE>(md, sp, (lds_weights)lds, S.net, FORCES, true, first)
+                          : network<WIDE>(md, sp, (global_weights)md.W, S.net, FORCES, true, first);
       }
     }
     const SpeciesDev &sp = *S.spc;
@@ -826,11 +826,14 @@ static void launch_one(const ModelDev &md, const BatchDev &b, const NbDev &nb, c
   P.w_in_lds = ps.w_in_lds;
   P.waves = ps.waves;
   if (MODE == 0 && P.tile_net && !o.desc_only)
-    hipLaunchKernelGGL((k_atom_cheb<RN, AN, MODE, F, MODE == 0>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream,
+    hipLaunchKernelGGL((k_atom_cheb<RN, AN, MODE, F, MODE == 0, false>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds,
+                       stream, md, b, nb, o, e, P);
+  else if (P.hs > 64)  // hidden layers wider than the wavefront
+    hipLaunchKernelGGL((k_atom_cheb<RN, AN, MODE, F, false, true>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream,
                        md, b, nb, o, e, P);
   else
-    hipLaunchKernelGGL((k_atom_cheb<RN, AN, MODE, F, false>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds, stream, md,
-                       b, nb, o, e, P);
+    hipLaunchKernelGGL((k_atom_cheb<RN, AN, MODE, F, false, false>), dim3(ps.grid), dim3(64 * ps.waves), ps.lds,
+                       stream, md, b, nb, o, e, P);
 }
 
 template <int RN, int AN>

@@ -149,8 +149,9 @@ __device__ inline NetLds carve_net(float *p, int nsf, int nl, int hs) {
   return n;
 }
 
-// layer stride for a model whose widest non-input layer has hmax nodes
-__host__ __device__ inline int net_stride(int hmax) { return hmax <= 32 ? 32 : AENET_AMD_MAX_HIDDEN; }
+// layer stride for a model whose widest non-input layer has hmax nodes:
+// 32, 64, then multiples of 32
+__host__ __device__ inline int net_stride(int hmax) { return hmax <= 32 ? 32 : hmax <= 64 ? 64 : (hmax + 31) / 32 * 32; }
 
 // A species record copied into a wavefront's LDS once per central atom:
 // its fields (layer sizes, offsets, typespin table, ...) are then LDS reads.
@@ -213,7 +214,8 @@ __device__ inline void first_layer_from_lanes(const SpeciesDev &sp, WPtr W, int 
   wave_sync();
 }
 
-// Feed-forward network of one species on the normalized descriptor in
+// Feed-forward network of one species (hidden layers wider than 64 nodes
+// need WIDE) on the normalized descriptor in
 // N.gp: ff_eval (src/ext/feedforward.f90:586-655), lanes = nodes of a
 // layer.  With first = 1 the first layer's outputs are already in N.act
 // (first_layer_from_lanes) and the forward pass starts from them.  With grad, also ff_deriv (:661-757) and the chain through the
@@ -222,7 +224,7 @@ __device__ inline void first_layer_from_lanes(const SpeciesDev &sp, WPtr W, int 
 // scale_in_coef, N.coef[f] holds the normalization scale of feature f on
 // entry (left there by the caller's normalization) instead of it being read
 // from global memory again.
-template <class WPtr>
+template <bool WIDE = false, class WPtr>
 __device__ inline float network(const ModelDev &md, const SpeciesDev &sp, WPtr W, const NetLds &N, bool grad,
                                 bool scale_in_coef = false, int first = 0) {
   const int lane = lane_id();
@@ -232,34 +234,41 @@ __device__ inline float network(const ModelDev &md, const SpeciesDev &sp, WPtr W
     const int n1 = sp.nnodes[l], n2 = sp.nnodes[l + 1];
     const WPtr Wl = W + sp.w_off[l];
     // lanes = (node, part): with NP the node count rounded up to a power of
-    // two (n2 <= 64), the 64 / NP parts share the inputs and a butterfly over
-    // the part bits of the lane sums them
+    // two (at most 64), the 64 / NP parts share the inputs and a butterfly
+    // over the part bits of the lane sums them; with WIDE, a layer wider
+    // than the wavefront takes its nodes 64 at a time (one part, every
+    // input).  Without WIDE (every layer at most 64 nodes) the node loop is
+    // one pass, and the kernels that run common networks keep their
+    // registers for the rest.
     // (shifts: NP is not known at compile time, and a division by it
     // would be a full integer division)
     int lg = 0;
-    while ((1 << lg) < n2) ++lg;
+    while ((1 << lg) < n2 && lg < 6) ++lg;
     const int NP = 1 << lg;
-    const int node = lane & (NP - 1), part = lane >> lg;
+    const int part = lane >> lg;
     const int per = (n1 * NP + WAVE - 1) >> 6;
     const int c0 = min(n1, part * per), c1 = min(n1, c0 + per);
-    float z0 = 0.0f, z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;
-    if (node < n2) {
-      int c = c0;
-      for (; c + 3 < c1; c += 4) {
-        z0 += Wl[c * n2 + node] * in[c];
-        z1 += Wl[(c + 1) * n2 + node] * in[c + 1];
-        z2 += Wl[(c + 2) * n2 + node] * in[c + 2];
-        z3 += Wl[(c + 3) * n2 + node] * in[c + 3];
+    for (int nb = 0; nb < (WIDE ? n2 : 1); nb += NP) {
+      const int node = nb + (lane & (NP - 1));
+      float z0 = 0.0f, z1 = 0.0f, z2 = 0.0f, z3 = 0.0f;
+      if (node < n2) {
+        int c = c0;
+        for (; c + 3 < c1; c += 4) {
+          z0 += Wl[c * n2 + node] * in[c];
+          z1 += Wl[(c + 1) * n2 + node] * in[c + 1];
+          z2 += Wl[(c + 2) * n2 + node] * in[c + 2];
+          z3 += Wl[(c + 3) * n2 + node] * in[c + 3];
+        }
+        for (; c < c1; ++c) z0 += Wl[c * n2 + node] * in[c];
       }
-      for (; c < c1; ++c) z0 += Wl[c * n2 + node] * in[c];
-    }
-    float z = (z0 + z1) + (z2 + z3);
-    for (int o = WAVE / 2; o >= NP; o >>= 1) z += __shfl_xor(z, o, WAVE);
-    if (part == 0 && node < n2) {
-      float y, dy;
-      activate(sp.fa[l + 1], z + Wl[n1 * n2 + node], y, dy);  // bias column
-      N.act[(l + 1) * N.hs + node] = y;
-      N.dact[(l + 1) * N.hs + node] = dy;
+      float z = (z0 + z1) + (z2 + z3);
+      for (int o = WAVE / 2; o >= NP; o >>= 1) z += __shfl_xor(z, o, WAVE);
+      if (part == 0 && node < n2) {
+        float y, dy;
+        activate(sp.fa[l + 1], z + Wl[n1 * n2 + node], y, dy);  // bias column
+        N.act[(l + 1) * N.hs + node] = y;
+        N.dact[(l + 1) * N.hs + node] = dy;
+      }
     }
     wave_sync();
     in = N.act + (l + 1) * N.hs;

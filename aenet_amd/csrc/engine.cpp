@@ -29,7 +29,8 @@ int DeviceModel::upload(const PotSet &ps, hipStream_t s) {
     nl_max = std::max(nl_max, sd.nlayers);
     for (int l = 1; l < sd.nlayers; ++l) hmax = std::max(hmax, sd.nnodes[l]);
   }
-  net_uniform = !ps.species.empty();
+  // the matrix-tile network: one architecture, hidden layers up to 64 nodes
+  net_uniform = !ps.species.empty() && hmax <= 64;
   for (const SpeciesDev &sd : ps.species) {
     const SpeciesDev &s0 = ps.species[0];
     bool same = sd.nlayers == s0.nlayers && sd.nlayers >= 2 && sd.nnodes[sd.nlayers - 1] == 1;

@@ -170,7 +170,7 @@ int read_nn(const std::string &path, PotHost &pot, std::string &err) {
   }
   for (int l = 0; l + 1 < n.nlayers; ++l)
     if (n.nnodes[l + 1] > AENET_AMD_MAX_HIDDEN && l + 1 < n.nlayers - 1) {
-      err = path + ": hidden layers wider than 64 nodes are not supported";
+      err = path + ": hidden layers wider than 512 nodes are not supported";
       return AENET_STATUS_ERR_IO;
     }
   if (n.nnodes[n.nlayers - 1] != 1 || n.nnodes[0] != s.nsf) {

@@ -60,7 +60,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) void k_net_grad(ModelDev md, N
     const SpeciesDev &sp = md.species[ti - 1];
     for (int f = lane; f < sp.nsf; f += WAVE) N.gp[f] = (float)a.G[(size_t)i * a.stride + f];
     wave_sync();
-    const float y = network(md, sp, (global_weights)md.W, N, false);
+    const float y = network<true>(md, sp, (global_weights)md.W, N, false);
     const double E = atom_energy(sp, y);
     double g;
     if (a.E_ref) {

@@ -459,7 +459,7 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_atom_cheb_alloy4000_traffic.json")
     tile = bool(tune and tune["chosen"].get("tile_network"))
-    kname = "k_atom_cheb<19,7,0,true,%s>" % ("true" if tile else "false")
+    kname = "k_atom_cheb<19,7,0,true,%s,false>" % ("true" if tile else "false")
     if args.replicas == 1 and os.path.exists(tpath):
         t = json.load(open(tpath))["kernels"].get(kname)
         if t:

@@ -41,6 +41,7 @@ def cases():
         ("mixed", ["A", "B"], mixed_structures()["bulk108"]),
         ("mixed", ["A", "B"], mixed_structures()["cluster40"]),
         ("chebhi", ["A", "B"], synth.binary_fcc(3.61, (3, 3, 3), 0.1, 19, 20)),
+        ("chebwide", ["A", "B"], synth.binary_fcc(3.61, (3, 3, 3), 0.1, 23, 24)),
     ]
 
 

@@ -174,6 +174,14 @@ def chebhi_set():
     return cheb_set("chebhi", ["A", "B"], [-1000.0, -1500.0], 6.0, 10, 4.0, 12, (16, 16), 70, cal)
 
 
+def chebwide_set():
+    """A/B with typespin, radial order 8 / 5.0 A and angular order 4 / 3.5 A
+    (28 features), 28-100-80-1: hidden layers wider than the 64 lanes of a
+    wavefront."""
+    cal = [synth.binary_fcc(3.61, (3, 3, 3), 0.1, 21, 22)]
+    return cheb_set("chebwide", ["A", "B"], [-1000.0, -1500.0], 5.0, 8, 3.5, 4, (100, 80), 80, cal)
+
+
 def main():
     os.makedirs(OUT, exist_ok=True)
     if sys.argv[1:] == ["mixed"]:
@@ -183,6 +191,10 @@ def main():
     if sys.argv[1:] == ["chebhi"]:
         chebhi_set()
         print("wrote the chebhi set to", OUT)
+        return
+    if sys.argv[1:] == ["chebwide"]:
+        chebwide_set()
+        print("wrote the chebwide set to", OUT)
         return
     # fcc108: single species, Chebyshev radial 16 / 6.0 A, angular 4 / 4.0 A, 22-20-20-1
     c, p = synth.fcc(3.61, (3, 3, 3))
@@ -214,6 +226,8 @@ def main():
     mixed_set()
     # Chebyshev with an angular order above the radial one
     chebhi_set()
+    # Chebyshev with hidden layers wider than a wavefront
+    chebwide_set()
     print("wrote potentials to", OUT)
 
 

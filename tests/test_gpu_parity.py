@@ -142,11 +142,11 @@ def test_neighbor_lists_at_the_cutoff(device):
         ann.release()
 
 
-@pytest.mark.parametrize("case", ["ref_01_alloy", "ref_03_bp3_fcc"])
+@pytest.mark.parametrize("case", ["ref_01_alloy", "ref_03_bp3_fcc", "ref_11_chebwide"])
 def test_per_atom_api_matches_reference(case, device):
     """aenet_atomic_energy_and_forces on the reference's own neighbor lists
     (the per-atom drop-in entry point, src/aenet.f90:410-486), Chebyshev
-    and Behler-Parrinello."""
+    and Behler-Parrinello, and hidden layers wider than a wavefront."""
     from aenet_amd.core import ANNPotentials
     path = [p for p in FIXTURES if os.path.basename(p).startswith(case)][0]
     g = _load(path)

@@ -28,21 +28,24 @@ def _model_and_batch(device, name="alloy", types=("A", "B"), structs=None):
     return m, db
 
 
-def test_weight_gradient_matches_oracle(device):
+@pytest.mark.parametrize("name", ["alloy", "chebwide"])
+def test_weight_gradient_matches_oracle(name, device):
+    """Weight gradients of single atomic energies against the oracle's
+    ff_wderiv restatement; chebwide has hidden layers of 100 and 80 nodes."""
     import torch
     from aenet_amd import nnfile
     from oracle_lib import Oracle
     names = ["A", "B"]
-    m, db = _model_and_batch(device)
+    m, db = _model_and_batch(device, name=name)
     G = db.descriptors(m.nsf_max, scaled=True)
     types = db.types
     n = db.natoms
-    o = Oracle(names, pot_files("alloy", names))
+    o = Oracle(names, pot_files(name, names))
     picks = [0, 1, 7, n - 1]
     for i in picks:
         t = int(types[i].item())
         off, size = m.species_weights(t)
-        escale = nnfile.read_nn(pot_files("alloy", names)[names[t - 1]])["E_scale"]
+        escale = nnfile.read_nn(pot_files(name, names)[names[t - 1]])["E_scale"]
         up = torch.zeros(n, dtype=torch.float64, device=device)
         up[i] = 1.0
         grad = torch.zeros(m.wsize, dtype=torch.float32, device=device)

@@ -37,6 +37,8 @@ def _structures():
         out.append(("mixed", ["A", "B"], s))
     # Chebyshev radial order 10, angular order 12
     out.append(("chebhi", ["A", "B"], synth.binary_fcc(3.61, (3, 3, 3), 0.1, 19, 20)))
+    # Chebyshev with hidden layers of 100 and 80 nodes
+    out.append(("chebwide", ["A", "B"], synth.binary_fcc(3.61, (3, 3, 3), 0.1, 23, 24)))
     return out
 
 
