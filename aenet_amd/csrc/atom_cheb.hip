@@ -410,6 +410,10 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
         if (r <= rRc && r > EPS_R) {
           float fc, dfc;
           cutoff(r, rRc, inv_rRc, fc, dfc);
+          // kept for the radial force pass, in the words of the neighbor's
+          // force, which that pass writes afterwards
+          S.fs[3 * q] = fc;
+          S.fs[3 * q + 1] = dfc;
           float x = 2.0f * r * inv_rRc - 1.0f;
           float sfc = sp.spin[S.t[q] - 1] * fc;
           float Tm = 1.0f, T = x;
@@ -593,8 +597,7 @@ __global__ __launch_bounds__(64 * CHEB_MAX_WAVES, 1) void k_atom_cheb(ModelDev m
       float gx = 0.0f, gy = 0.0f, gz = 0.0f;
       float r = S.r[q];
       if (r <= rRc && r > EPS_R) {
-        float fc, dfc;
-        cutoff(r, rRc, inv_rRc, fc, dfc);
+        const float fc = fs[3 * q], dfc = fs[3 * q + 1];  // from the descriptor pass
         float x = 2.0f * r * inv_rRc - 1.0f;
         float sj = multi ? sp.spin[S.t[q] - 1] : 0.0f;
         float fcx = fc * 2.0f * inv_rRc;

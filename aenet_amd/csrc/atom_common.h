@@ -200,9 +200,18 @@ __device__ inline void first_layer_from_lanes(const SpeciesDev &sp, WPtr W, int 
   const int lane = lane_id();
   const int n1 = sp.nnodes[0], n2 = sp.nnodes[1];
   const WPtr W0 = W + sp.w_off[0];
+  // Branch-free, at immediate offsets: every lane reads 32 words from the
+  // start of its row (row 0 for a lane without a feature, with x = 0).
+  // Words j >= n2 lie past the row (the next rows, the bias column, the
+  // next layer or species; the device copy of the weights is padded by 32
+  // words, and the staged copy is followed by more of the workgroup's LDS);
+  // their products only enter the sums of columns j >= n2, which belong to
+  // no node and are never read.
+  const WPtr row = W0 + (f >= 0 ? f : 0) * n2;
+  const float xf = f >= 0 ? x : 0.0f;
   float v[32];
 #pragma unroll
-  for (int j = 0; j < 32; ++j) v[j] = (f >= 0 && j < n2) ? W0[f * n2 + j] * x : 0.0f;
+  for (int j = 0; j < 32; ++j) v[j] = row[j] * xf;
   const float z = rs_levels<32, 32>(v);
   const int node = lane >> 1;
   if ((lane & 1) == 0 && node < n2) {

@@ -38,7 +38,12 @@ int DeviceModel::upload(const PotSet &ps, hipStream_t s) {
     if (!same) net_uniform = 0;
   }
   if (species.upload(ps.species.data(), ps.species.size(), s)) return AENET_STATUS_ERR_DEVICE;
-  if (W.upload(ps.W.data(), ps.W.size(), s)) return AENET_STATUS_ERR_DEVICE;
+  // 32 words of padding: the first layer's products read up to 31 words
+  // past a row (first_layer_from_lanes); the copy lives until the stream
+  // synchronization below
+  std::vector<float> wpad(ps.W);
+  wpad.resize(ps.W.size() + 32, 0.0f);
+  if (W.upload(wpad.data(), wpad.size(), s)) return AENET_STATUS_ERR_DEVICE;
   if (avg.upload(ps.norm_avg.data(), ps.norm_avg.size(), s)) return AENET_STATUS_ERR_DEVICE;
   if (scale.upload(ps.norm_scale.data(), ps.norm_scale.size(), s)) return AENET_STATUS_ERR_DEVICE;
   if (bp.upload(&ps.bp, 1, s)) return AENET_STATUS_ERR_DEVICE;

@@ -91,7 +91,9 @@ template <int O>
 __device__ __forceinline__ unsigned partner_u32(unsigned x) {
   static_assert(O == 8 || O == 4 || O == 2 || O == 1, "DPP within a row");
   constexpr int ctrl = O == 8 ? 0x128 : O == 4 ? 0x141 : O == 2 ? 0x4E : 0xB1;
-  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)x, ctrl, 0xf, 0xf, false);
+  // every lane's source is inside its row for these patterns, so no lane
+  // keeps the old value and none needs to be set first
+  return (unsigned)__builtin_amdgcn_mov_dpp((int)x, ctrl, 0xf, 0xf, false);
 }
 
 template <int O>
@@ -131,6 +133,31 @@ __device__ __forceinline__ float wave_sum(float v) { return butterfly_sum<32>(v)
 __device__ __forceinline__ double wave_sum(double v) { return butterfly_sum<32>(v); }
 
 __device__ __forceinline__ long long wave_sum(long long v) { return butterfly_sum<32>(v); }
+
+// The wavefront sums of three per-lane integers at once, returned in lane 0
+// (other lanes: partial sums): two reduce-scatter levels leave the three
+// sums' partials in lanes 0-15, 32-47 and 16-31, a butterfly over each
+// 16-lane row finishes them, and lane 0 reads the other two rows' results.
+// Integer addition, so the result is the same as three wave_sum calls.
+__device__ __forceinline__ void wave_sum3_lane0(long long (&v)[3]) {
+  long long a = v[0], b = v[1], c = v[2], d = 0;
+  half_swap<32>(a, b);
+  a += b;  // lanes 0-31: v[0] over lanes l, l+32; lanes 32-63: v[1]
+  half_swap<32>(c, d);
+  c += d;  // lanes 0-31: v[2]; lanes 32-63: zero
+  half_swap<16>(a, c);
+  a += c;  // rows: v[0], v[2], v[1], zero
+  a = butterfly_sum<8>(a);
+  const unsigned long long u = (unsigned long long)a;
+  const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+  auto row = [&](int l) {
+    return (long long)(((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)hi, l) << 32) |
+                       (unsigned)__builtin_amdgcn_readlane((int)lo, l));
+  };
+  v[0] = a;
+  v[1] = row(32);
+  v[2] = row(16);
+}
 
 // Forces of structures are summed as 64-bit fixed-point integers in units
 // of 2^-32 eV/A: integer addition is associative, so the sum of the
@@ -190,7 +217,7 @@ __device__ inline void scatter_forces(double *F, int *fmark, int i, int n, const
       bad = true;
     }
   }
-  for (int x = 0; x < 3; ++x) own[x] = wave_sum(own[x]);
+  wave_sum3_lane0(own);
   bad = wave_ballot(bad) != 0;
   if (lane_id() == 0) {
     if (bad)
@@ -259,7 +286,7 @@ __device__ inline void scatter_forces_table(const ForceTable &t, double *F, int 
       bad = true;
     }
   }
-  for (int x = 0; x < 3; ++x) own[x] = wave_sum(own[x]);
+  wave_sum3_lane0(own);
   bad = wave_ballot(bad) != 0;
   if (lane_id() == 0) {
     if (bad)
