@@ -60,6 +60,9 @@ struct BPDev {
   // ang_p0, sign-sorted) with ang_np of lambda = +1 first
   int ang_fast[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];
   int ang_np[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];
+  // fast-path block whose functions are every combination of a set of
+  // (lambda, zeta) pairs with a set of etas: 1 + its BPAngGrid record (0: none)
+  int ang_grid[AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES * AENET_AMD_MAX_TYPES];
 };
 
 // one radial (G1/G2/G3) or angular (G4/G5) function record
@@ -138,6 +141,25 @@ struct NetGradArgs {
   int *flags;              // optional error word
 };
 
+// An angular block whose G4 (or G5) functions are the nz x ne combinations
+// of nz (lambda = +-1, zeta) pairs and ne etas, at one cutoff.  With arg =
+// (1 + lambda cos) / 2 the function of pair iz and eta ie factorizes,
+//   G = 2 arg^zeta exp(-eta S) C = Z_iz E_ie,
+//   Z_iz = 2^(zeta log2(arg) + 1),  E_ie = 2^(-eta log2(e) S) C,
+// so a triplet takes nz + ne exponentials instead of one per function.
+// Slot s = iz BP_GRID_NE + ie holds the function's place in the block.
+constexpr int BP_GRID_NZ = 8;
+constexpr int BP_GRID_NE = 4;
+struct BPAngGrid {
+  int nz, ne;
+  float zeta[BP_GRID_NZ];
+  float lam[BP_GRID_NZ];   // +1 or -1
+  float K1[BP_GRID_NZ];    // zeta lambda / 2
+  float B[BP_GRID_NE];     // -eta log2(e)
+  float K2[BP_GRID_NE];    // -2 eta
+  int feat[BP_GRID_NZ * BP_GRID_NE];  // place in the block (feature ang_off + feat), -1: none
+};
+
 struct ModelDev {
   const SpeciesDev *species;  // [ntypes]
   const float *W;             // network weights
@@ -146,6 +168,7 @@ struct ModelDev {
   const BPDev *bp;
   const BPFunc *bpf;
   const BPAngK *bpk;  // parallel to bpf (angular fast-path blocks only)
+  const BPAngGrid *bpg;  // grid blocks (BPDev::ang_grid)
   int ntypes;
   int desc;
 };

@@ -56,6 +56,7 @@ constexpr int MT = AENET_AMD_MAX_TYPES;
 typedef const __attribute__((address_space(4))) BPFunc *bp_funcs;
 typedef const __attribute__((address_space(4))) BPDev *bp_tables;
 typedef const __attribute__((address_space(4))) BPAngK *bp_angk;
+typedef const __attribute__((address_space(4))) BPAngGrid *bp_grid;
 
 // 2^x and log2(x) as the hardware instructions (1 ulp).  The library forms
 // add scaling for subnormal inputs and results only: the log2 arguments here
@@ -550,6 +551,109 @@ struct AngForceFastOp {
   }
 };
 
+// energy pass, grid path (BPAngGrid): per triplet E_ie = 2^(B_ie S) C for
+// the ne etas and Z_iz = 2^(zeta_iz log2(arg) + 1) for the nz (lambda,
+// zeta) pairs; slot iz BP_GRID_NE + ie adds Z_iz E_ie (slots past ne add
+// zero)
+static_assert(BP_GRID_NZ * BP_GRID_NE == ACH, "one register chunk of grid slots");
+struct AngValueGridOp {
+  const float *R, *r;
+  CutTab tab;
+  bp_grid G;
+  float rc;
+  bool g4;
+  float acc[ACH];
+  __device__ __forceinline__ void operator()(int ja, int kb, bool valid) {
+    if (!valid) return;
+    const Trip T = make_trip(R, r, ja, kb);
+    AngFastGeom q;
+    fast_geom(T, g4, rc, tab, ja, kb, q);
+    const int nz = G->nz, ne = G->ne;
+    float E[BP_GRID_NE];
+#pragma unroll
+    for (int ie = 0; ie < BP_GRID_NE; ++ie) E[ie] = ie < ne ? exp2_hw(G->B[ie] * q.S) * q.C : 0.0f;
+#pragma unroll
+    for (int iz = 0; iz < BP_GRID_NZ; ++iz) {
+      if (iz < nz) {
+        const float Z = exp2_hw(fmaf(G->zeta[iz], G->lam[iz] > 0.0f ? q.Lp : q.Lm, 1.0f));
+#pragma unroll
+        for (int ie = 0; ie < BP_GRID_NE; ++ie) acc[iz * BP_GRID_NE + ie] = fmaf(Z, E[ie], acc[iz * BP_GRID_NE + ie]);
+      }
+    }
+  }
+};
+
+// force pass, grid path: the sums of the fast path's force op by pairs,
+//   u0_iz = sum_ie c_s E_ie,  u2_iz = sum_ie c_s K2_ie E_ie  (c_s = dE/dG of
+//   slot s, E_ie = 2^(B_ie S) without the cutoffs),
+// then with Zm_iz = 2^((zeta_iz - 1) log2(arg)): s_cos = 2 sum K1 Zm u0, and
+// per sign of lambda sum Zm u0 and sum Zm u2
+struct AngForceGridOp {
+  const float *R, *r;
+  CutTab tab;
+  float *fs;
+  bp_grid G;
+  float cs[ACH];  // dE/dG per slot (0: no function)
+  float rc;
+  bool g4;
+  __device__ __forceinline__ void operator()(int ja, int kb, bool valid) {
+    if (!valid) return;
+    const Trip T = make_trip(R, r, ja, kb);
+    AngFastGeom q;
+    fast_geom(T, g4, rc, tab, ja, kb, q);
+    const int nz = G->nz, ne = G->ne;
+    float E[BP_GRID_NE], EK[BP_GRID_NE];
+#pragma unroll
+    for (int ie = 0; ie < BP_GRID_NE; ++ie) {
+      E[ie] = ie < ne ? exp2_hw(G->B[ie] * q.S) : 0.0f;
+      EK[ie] = G->K2[ie] * E[ie];
+    }
+    float sc = 0.0f, sep = 0.0f, sem = 0.0f, shp = 0.0f, shm = 0.0f;
+#pragma unroll
+    for (int iz = 0; iz < BP_GRID_NZ; ++iz) {
+      if (iz < nz) {
+        float u0 = 0.0f, u2 = 0.0f;
+#pragma unroll
+        for (int ie = 0; ie < BP_GRID_NE; ++ie) {
+          u0 = fmaf(cs[iz * BP_GRID_NE + ie], E[ie], u0);
+          u2 = fmaf(cs[iz * BP_GRID_NE + ie], EK[ie], u2);
+        }
+        const bool pos = G->lam[iz] > 0.0f;
+        const float Zm = exp2_hw((G->zeta[iz] - 1.0f) * (pos ? q.Lp : q.Lm));
+        const float t0 = Zm * u0, t2 = Zm * u2;
+        sc = fmaf(G->K1[iz], t0, sc);
+        if (pos) {
+          sep += t0;
+          shp += t2;
+        } else {
+          sem += t0;
+          shm += t2;
+        }
+      }
+    }
+    const float s_cos = 2.0f * sc;
+    const float s_E = 2.0f * (q.argp * sep + q.argm * sem);
+    const float s_eta = 2.0f * (q.argp * shp + q.argm * shm);
+    const TripCut &c = q.c;
+    const float cjk = g4 ? c.cjk : 1.0f;
+    const float C = q.C;
+    const float dcos = s_cos * C;
+    const float dRj = s_E * (c.dcj * c.ck * cjk) + s_eta * T.rj * C;
+    const float dRk = s_E * (c.cj * c.dck * cjk) + s_eta * T.rk * C;
+    const float dRjk = g4 ? s_E * (c.cj * c.ck * c.dcjk) + s_eta * T.rjk * C : 0.0f;
+    const float inv_rjk = T.rjk > 1.0e-8f ? __builtin_amdgcn_rcpf(T.rjk) : 1.0f;
+    const float cj = dcos * T.ij, ck = dcos * T.ik;
+#pragma unroll
+    for (int x = 0; x < 3; ++x) {
+      const float e = dRjk * T.d[x] * inv_rjk;
+      const float gj = dRj * T.uj[x] + cj * (T.uk[x] - T.cosv * T.uj[x]) - e;
+      const float gk = dRk * T.uk[x] + ck * (T.uj[x] - T.cosv * T.uk[x]) + e;
+      atomicAdd(&fs[3 * ja + x], gj);
+      atomicAdd(&fs[3 * kb + x], gk);
+    }
+  }
+};
+
 // MODE 0: structures (compact lists, fixed-point atomic force scatter)
 // MODE 1: explicit environments (per-neighbor forces out)
 // Registers are held to what BP_SIMD_WAVES wavefronts per SIMD allow.
@@ -741,7 +845,30 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) __attribute__((amdgpu_waves_pe
         };
         const bool fast = bpt->ang_fast[key3] != 0;
         const bp_angk K = (bp_angk)md.bpk + bpt->ang_p0[key3];
-        for (int c0 = 0; c0 < nf; c0 += ACH) {
+        const int gidx = bpt->ang_grid[key3];
+        if (gidx > 0) {
+          // one chunk: the grid's slots
+          AngValueGridOp op;
+          op.R = S.R;
+          op.r = S.r;
+          op.tab = tab;
+          op.G = (bp_grid)md.bpg + (gidx - 1);
+          op.rc = fn[0].Rc;
+          op.g4 = fn[0].kind == 4;
+#pragma unroll
+          for (int k = 0; k < ACH; ++k) op.acc[k] = 0.0f;
+          traverse(op, 0);
+          const double G = reduce_scatter<ACH>(op.acc);
+          const int slot = lane >> 1;
+          const int fb = op.G->feat[slot];
+          if ((lane & 1) == 0 && fb >= 0) {
+            const int ft = off + fb;
+            const double Gs = md.norm_scale[sp.norm_off + ft] * (G - md.norm_avg[sp.norm_off + ft]);
+            S.net.gp[ft] = (float)Gs;
+            if (o.G) o.G[(size_t)g * o.G_stride + ft] = o.G_scaled ? Gs : G;
+          }
+        }
+        for (int c0 = 0; c0 < (gidx > 0 ? 0 : nf); c0 += ACH) {
           double G;
           if (fast) {
             AngValueFastOp op;
@@ -869,7 +996,22 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) __attribute__((amdgpu_waves_pe
             for_triplets(S, b2, m2, b3, m3, t2 == t3, bpt->ang_g5[key3] != 0, rc * rc, op);
         };
         const bp_funcs fn = bpf + bpt->ang_p0[key3];
-        if (bpt->ang_fast[key3]) {
+        if (bpt->ang_grid[key3] > 0) {
+          AngForceGridOp op;
+          op.R = S.R;
+          op.r = S.r;
+          op.tab = tab;
+          op.fs = S.fs;
+          op.G = (bp_grid)md.bpg + (bpt->ang_grid[key3] - 1);
+#pragma unroll
+          for (int s2 = 0; s2 < ACH; ++s2) {
+            const int fb = op.G->feat[s2];
+            op.cs[s2] = fb >= 0 ? coef[off + fb] : 0.0f;
+          }
+          op.rc = fn[0].Rc;
+          op.g4 = fn[0].kind == 4;
+          traverse(op);
+        } else if (bpt->ang_fast[key3]) {
           AngForceFastOp op;
           op.R = S.R;
           op.r = S.r;

@@ -49,6 +49,7 @@ int DeviceModel::upload(const PotSet &ps, hipStream_t s) {
   if (bp.upload(&ps.bp, 1, s)) return AENET_STATUS_ERR_DEVICE;
   if (bpf.upload(ps.bpf.data(), ps.bpf.size(), s)) return AENET_STATUS_ERR_DEVICE;
   if (bpk.upload(ps.bpk.data(), ps.bpk.size(), s)) return AENET_STATUS_ERR_DEVICE;
+  if (bpg.upload(ps.bpg.data(), ps.bpg.size(), s)) return AENET_STATUS_ERR_DEVICE;
   if (hipStreamSynchronize(s) != hipSuccess) return AENET_STATUS_ERR_DEVICE;
   return AENET_STATUS_OK;
 }
@@ -62,6 +63,7 @@ ModelDev DeviceModel::view() const {
   m.bp = bp.p;
   m.bpf = bpf.p;
   m.bpk = bpk.p;
+  m.bpg = bpg.p;
   m.ntypes = ntypes;
   m.desc = desc;
   return m;

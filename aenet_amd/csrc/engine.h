@@ -90,6 +90,7 @@ struct DeviceModel {
   DBuf<BPDev> bp;
   DBuf<BPFunc> bpf;
   DBuf<BPAngK> bpk;
+  DBuf<BPAngGrid> bpg;
   int ntypes = 0, desc = 0, max_rN = 0, max_aN = 0, nsf_max = 0, wsize = 0, nl_max = 0, hmax = 1, nnorm = 0, net_uniform = 0;
   double rc_max = 0.0;
   AtomSizes sizes(int natoms, int cap, const int *active = nullptr, const int *nactive = nullptr) const {

@@ -3,6 +3,7 @@
 // File layout: load_Network (src/ext/feedforward.f90:291-306), load_Setup
 // (src/sfsetup.f90:585-609), load_TrnSet_info (src/trainset.f90:430-440);
 // derived energy constants as load_NNPot (src/potential.f90:145-151).
+#include <algorithm>
 #include "model.h"
 
 #include <cmath>
@@ -307,6 +308,7 @@ int PotSet::finalize(std::string &err) {
     bp.ntypes = ntypes;
     bpf.clear();
     bpk.clear();
+    bpg.clear();
     // registration lists per (row, funct, t2[, t3])
     for (int row = 0; row < ntypes; ++row) {
       const SetupHost &s = pots[row].stp;
@@ -375,6 +377,47 @@ int PotSet::finalize(std::string &err) {
               bp.ang_fast[key3] = 1;
             }
           }
+          // the grid path: a fast block whose functions are every
+          // combination of its (lambda, zeta) pairs and its etas, once each
+          bp.ang_grid[key3] = 0;
+          if (bp.ang_fast[key3]) {
+            const int p0 = bp.ang_p0[key3], n = bp.ang_n[key3];
+            std::vector<std::pair<float, float>> lz;
+            std::vector<float> et;
+            for (int k = p0; k < p0 + n; ++k) {
+              const std::pair<float, float> v(bpf[k].p1, bpf[k].p2);
+              if (std::find(lz.begin(), lz.end(), v) == lz.end()) lz.push_back(v);
+              if (std::find(et.begin(), et.end(), bpf[k].p3) == et.end()) et.push_back(bpf[k].p3);
+            }
+            if ((int)lz.size() <= BP_GRID_NZ && (int)et.size() <= BP_GRID_NE && (int)(lz.size() * et.size()) == n) {
+              BPAngGrid g{};
+              g.nz = (int)lz.size();
+              g.ne = (int)et.size();
+              for (int q = 0; q < BP_GRID_NZ * BP_GRID_NE; ++q) g.feat[q] = -1;
+              bool once = true;
+              for (int k = p0; k < p0 + n; ++k) {
+                const int iz = (int)(std::find(lz.begin(), lz.end(), std::make_pair(bpf[k].p1, bpf[k].p2)) - lz.begin());
+                const int ie = (int)(std::find(et.begin(), et.end(), bpf[k].p3) - et.begin());
+                int &slot = g.feat[iz * BP_GRID_NE + ie];
+                if (slot >= 0) once = false;
+                slot = k - p0;
+              }
+              for (int iz = 0; iz < g.nz; ++iz) {
+                g.lam[iz] = lz[iz].first;
+                g.zeta[iz] = lz[iz].second;
+                g.K1[iz] = 0.5f * lz[iz].second * lz[iz].first;
+              }
+              for (int ie = 0; ie < g.ne; ++ie) {
+                g.B[ie] = (float)(-(double)et[ie] * 1.4426950408889634);
+                g.K2[ie] = -2.0f * et[ie];
+              }
+              if (once) {
+                bpg.push_back(g);
+                bp.ang_grid[key3] = (int)bpg.size();
+              }
+            }
+          }
+          bp.ang_grid[key3b] = bp.ang_grid[key3];
           bp.ang_fast[key3b] = bp.ang_fast[key3];
           bp.ang_np[key3b] = bp.ang_np[key3];
           bp.ang_off[key3b] = bp.ang_off[key3];

@@ -65,6 +65,7 @@ struct PotSet {
   BPDev bp{};
   std::vector<BPFunc> bpf;
   std::vector<BPAngK> bpk;  // parallel to bpf
+  std::vector<BPAngGrid> bpg;  // grid blocks
   int max_rN = 0, max_aN = 0;
 
   // called when the last potential is loaded (aenet_load_potential,

@@ -55,7 +55,11 @@ def _bp3_variant(tmp_path, kind):
     of one cutoff, one kind and lambda = +-1, functions sign-sorted) meets
     the cases bp3 itself does not hold: 'g5' turns every G4 into G5 (no r_jk
     cutoff, S = r_ij^2 + r_ik^2); 'signs' gives the (A, A) block lambda = -1
-    only and the (B, B) block lambda = +1 only."""
+    only and the (B, B) block lambda = +1 only.  bp3's blocks are complete
+    grids of (lambda, zeta) pairs and etas, which the kernel evaluates by
+    factors (BPAngGrid); 'signs' makes two blocks repeat functions, and
+    'nogrid' moves the first G4 of every block to an eta of its own, so
+    that those blocks take the per-function fast path."""
     from aenet_amd.nnfile import read_nn, write_nn
     files = {}
     for t in "ABC":
@@ -65,6 +69,12 @@ def _bp3_variant(tmp_path, kind):
         env = np.array(p["sfenv"]).reshape(int(p["nsf"]), 2)
         if kind == "g5":
             sf[sf == 4] = 5
+        elif kind == "nogrid":
+            seen = set()
+            for i in range(len(sf)):
+                if sf[i] == 4 and tuple(sorted(env[i])) not in seen:
+                    seen.add(tuple(sorted(env[i])))
+                    par[i, 3] = 0.05
         else:
             for i in range(len(sf)):
                 if sf[i] == 4 and tuple(env[i]) == (1, 1):
@@ -79,7 +89,7 @@ def _bp3_variant(tmp_path, kind):
     return files
 
 
-@pytest.mark.parametrize("kind", ["g5", "signs"])
+@pytest.mark.parametrize("kind", ["g5", "signs", "nogrid"])
 def test_bp_angular_fast_path_variants(kind, tmp_path, device):
     """Energies and forces of the fast-path variants against the FP64
     oracle: a displaced 108-atom FCC cell of three species and a 40-atom
