@@ -211,7 +211,7 @@ def batched_bp(steps, warmup):
             "atoms_per_step": db.natoms,
             "ms_per_step": ms,
             "atoms_per_s": db.natoms / (ms * 1e-3),
-            "kernel": "k_atom_bp<0,true>",
+            "kernel": "k_atom_bp<0,true,false>",
             "kernel_ms": k_s * 1e3,
             "pairs": n_p,
             "contributing_triplets": n_t,
@@ -224,7 +224,7 @@ def batched_bp(steps, warmup):
         if plan is not None:
             # kernel_ms then covers the bucketing launch and one atom-kernel
             # launch per neighbor-count bucket
-            out["kernel"] = "k_bucket_atoms + k_atom_bp<0,true> per bucket"
+            out["kernel"] = "k_bucket_atoms + k_atom_bp<0,true,false> per bucket"
             out["neighbor_count_buckets"] = {
                 "caps": plan["caps"], "atoms": plan["atoms"],
                 "shapes": [{"waves_per_workgroup": w, "weights_in_lds": st, "tile_network": t}
