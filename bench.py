@@ -57,6 +57,7 @@ def _cpu_worker(args):
     lib.refdrv_timed.argtypes = [np.ctypeslib.ndpointer(np.float64), C.c_int, C.c_int,
                                  np.ctypeslib.ndpointer(np.int32), np.ctypeslib.ndpointer(np.float64), C.c_int,
                                  C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 np.ctypeslib.ndpointer(np.float64), np.ctypeslib.ndpointer(np.float64),
                                  C.POINTER(C.c_int)]
     n = len(s)
     frac = np.zeros((n, 3))
@@ -64,15 +65,17 @@ def _cpu_worker(args):
     lib.refdrv_cart_to_frac(np.ascontiguousarray(s.cell).ravel(), 1, n, np.ascontiguousarray(s.positions).ravel(),
                             frac, L)
     t_nb = t_ev = 0.0
+    E = np.full(n, np.nan)
+    F = np.zeros((n, 3))
     for _ in range(reps):
         tn, te, et = C.c_double(), C.c_double(), C.c_double()
         st = C.c_int()
         lib.refdrv_timed(L, 1, n, np.ascontiguousarray(s.types, np.int32), frac, rank, nranks, C.byref(tn),
-                         C.byref(te), C.byref(et), C.byref(st))
+                         C.byref(te), C.byref(et), E, F, C.byref(st))
         t_nb += tn.value
         t_ev += te.value
     count = reps * len(range(rank, n, nranks))
-    return count, t_nb, t_ev
+    return count, t_nb, t_ev, E, F
 
 
 def cpu_baseline(procs, stride, reps):
@@ -90,6 +93,14 @@ def cpu_baseline(procs, stride, reps):
         res = pool.map(_cpu_worker, [(r, nranks, reps) for r in range(procs)])
     wall = time.time() - t0
     atoms = sum(r[0] for r in res)
+    # the reference's per-atom energies of the sampled atoms and, when every
+    # atom is sampled, its forces (the workers' contributions summed)
+    import numpy as np
+    E_ref = np.full(len(res[0][3]), np.nan)
+    for r in res:
+        m = ~np.isnan(r[3])
+        E_ref[m] = r[3][m]
+    F_ref = sum(r[4] for r in res) if stride == 1 else None
     t_eval = max(r[2] for r in res)
     t_e2e = max(r[1] + r[2] for r in res)
     cpu_s = sum(r[1] + r[2] for r in res)
@@ -108,6 +119,8 @@ def cpu_baseline(procs, stride, reps):
         "host_cores_total": os.cpu_count(),
         "host_cores_affinity": sys_cores(),
         "cores_note": "worker processes capped at 16, the host CPU share of one GPU on these machines",
+        "_E_ref": E_ref,
+        "_F_ref": F_ref,
     }
 
 
@@ -522,6 +535,21 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if cpu is not None:
+        # the GPU step against the reference's own numbers for the sampled
+        # atoms (energies) and, when every atom is sampled, the forces
+        import numpy as np
+        E_ref, F_ref = cpu.pop("_E_ref", None), cpu.pop("_F_ref", None)
+        if E_ref is not None:
+            m = ~np.isnan(E_ref)
+            Eg = db.E_atoms.cpu().numpy()
+            cpu["parity"] = {
+                "atoms_compared": int(m.sum()),
+                "max_abs_dE_eV_per_atom": float(np.abs(Eg[m] - E_ref[m]).max()),
+                "max_abs_dF_eV_per_A": (float(np.abs(db.forces.cpu().numpy() - F_ref).max())
+                                        if F_ref is not None else None),
+                "bars": "1e-5 eV/atom, 1e-4 eV/A",
+            }
     if cpu and cpu.get("value"):
         out["gpu_over_cpu"] = value / cpu["value"]
         out["gpu_over_cpu_end_to_end"] = value / cpu["end_to_end"]

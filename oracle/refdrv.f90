@@ -154,7 +154,7 @@ contains
   ! (lcl_nbdist_cart), t_eval = aenet_atomic_energy_and_forces on the
   ! stored lists (the "kernel-equivalent" part); wall-clock seconds.
   subroutine refdrv_timed(latt, pbc, natoms, types, frac, rank, nranks, &
-                          t_nbl, t_eval, Etot, stat) bind(C, name='refdrv_timed')
+                          t_nbl, t_eval, Etot, E_out, F_out, stat) bind(C, name='refdrv_timed')
     real(c_double), dimension(3,3),              intent(in)    :: latt
     integer(c_int), value,                       intent(in)    :: pbc
     integer(c_int), value,                       intent(in)    :: natoms
@@ -165,6 +165,10 @@ contains
     real(c_double),                              intent(out)   :: t_nbl
     real(c_double),                              intent(out)   :: t_eval
     real(c_double),                              intent(out)   :: Etot
+    ! energies of this rank's atoms (others untouched) and the force sums of
+    ! their contributions to every atom (the reference's F of these calls)
+    real(c_double), dimension(natoms),           intent(inout) :: E_out
+    real(c_double), dimension(3,natoms),         intent(out)   :: F_out
     integer(c_int),                              intent(out)   :: stat
 
     integer,          dimension(:,:),   allocatable :: nblist, nbtype
@@ -209,9 +213,11 @@ contains
             nbcoo(:,1:nnb(k),k), nbtype(1:nnb(k),k), nblist(1:nnb(k),k), natoms, E_i, F, st)
        if (st /= 0) stat = st
        Etot = Etot + E_i
+       E_out(idx(k)) = E_i
     end do
     call system_clock(c1)
     t_eval = dble(c1 - c0)/dble(crate)
+    F_out = F
     deallocate(nblist, nbtype, nbcoo, nbdist, cooi, nnb, idx, F)
 
   end subroutine refdrv_timed
