@@ -152,6 +152,7 @@ constexpr int BP_GRID_NZ = 8;
 constexpr int BP_GRID_NE = 4;
 struct BPAngGrid {
   int nz, ne;
+  int nzp;                 // pairs with lambda = +1, which come first
   float zeta[BP_GRID_NZ];
   float lam[BP_GRID_NZ];   // +1 or -1
   float K1[BP_GRID_NZ];    // zeta lambda / 2

@@ -556,10 +556,34 @@ struct AngForceFastOp {
 // zeta) pairs; slot iz BP_GRID_NE + ie adds Z_iz E_ie (slots past ne add
 // zero)
 static_assert(BP_GRID_NZ * BP_GRID_NE == ACH, "one register chunk of grid slots");
+// The grid's constants, read once per block (not per triplet round)
+struct GridConst {
+  int nz, ne, nzp;
+  float zeta[BP_GRID_NZ];
+  float K1[BP_GRID_NZ];
+  float B[BP_GRID_NE];
+  float K2[BP_GRID_NE];
+  __device__ __forceinline__ void load(bp_grid G) {
+    nz = G->nz;
+    ne = G->ne;
+    nzp = G->nzp;
+#pragma unroll
+    for (int iz = 0; iz < BP_GRID_NZ; ++iz) {
+      zeta[iz] = G->zeta[iz];
+      K1[iz] = G->K1[iz];
+    }
+#pragma unroll
+    for (int ie = 0; ie < BP_GRID_NE; ++ie) {
+      B[ie] = G->B[ie];
+      K2[ie] = G->K2[ie];
+    }
+  }
+};
+
 struct AngValueGridOp {
   const float *R, *r;
   CutTab tab;
-  bp_grid G;
+  GridConst K;
   float rc;
   bool g4;
   float acc[ACH];
@@ -568,14 +592,13 @@ struct AngValueGridOp {
     const Trip T = make_trip(R, r, ja, kb);
     AngFastGeom q;
     fast_geom(T, g4, rc, tab, ja, kb, q);
-    const int nz = G->nz, ne = G->ne;
     float E[BP_GRID_NE];
 #pragma unroll
-    for (int ie = 0; ie < BP_GRID_NE; ++ie) E[ie] = ie < ne ? exp2_hw(G->B[ie] * q.S) * q.C : 0.0f;
+    for (int ie = 0; ie < BP_GRID_NE; ++ie) E[ie] = ie < K.ne ? exp2_hw(K.B[ie] * q.S) * q.C : 0.0f;
 #pragma unroll
     for (int iz = 0; iz < BP_GRID_NZ; ++iz) {
-      if (iz < nz) {
-        const float Z = exp2_hw(fmaf(G->zeta[iz], G->lam[iz] > 0.0f ? q.Lp : q.Lm, 1.0f));
+      if (iz < K.nz) {
+        const float Z = exp2_hw(fmaf(K.zeta[iz], iz < K.nzp ? q.Lp : q.Lm, 1.0f));
 #pragma unroll
         for (int ie = 0; ie < BP_GRID_NE; ++ie) acc[iz * BP_GRID_NE + ie] = fmaf(Z, E[ie], acc[iz * BP_GRID_NE + ie]);
       }
@@ -592,7 +615,7 @@ struct AngForceGridOp {
   const float *R, *r;
   CutTab tab;
   float *fs;
-  bp_grid G;
+  GridConst K;
   float cs[ACH];  // dE/dG per slot (0: no function)
   float rc;
   bool g4;
@@ -601,27 +624,26 @@ struct AngForceGridOp {
     const Trip T = make_trip(R, r, ja, kb);
     AngFastGeom q;
     fast_geom(T, g4, rc, tab, ja, kb, q);
-    const int nz = G->nz, ne = G->ne;
     float E[BP_GRID_NE], EK[BP_GRID_NE];
 #pragma unroll
     for (int ie = 0; ie < BP_GRID_NE; ++ie) {
-      E[ie] = ie < ne ? exp2_hw(G->B[ie] * q.S) : 0.0f;
-      EK[ie] = G->K2[ie] * E[ie];
+      E[ie] = ie < K.ne ? exp2_hw(K.B[ie] * q.S) : 0.0f;
+      EK[ie] = K.K2[ie] * E[ie];
     }
     float sc = 0.0f, sep = 0.0f, sem = 0.0f, shp = 0.0f, shm = 0.0f;
 #pragma unroll
     for (int iz = 0; iz < BP_GRID_NZ; ++iz) {
-      if (iz < nz) {
+      if (iz < K.nz) {
         float u0 = 0.0f, u2 = 0.0f;
 #pragma unroll
         for (int ie = 0; ie < BP_GRID_NE; ++ie) {
           u0 = fmaf(cs[iz * BP_GRID_NE + ie], E[ie], u0);
           u2 = fmaf(cs[iz * BP_GRID_NE + ie], EK[ie], u2);
         }
-        const bool pos = G->lam[iz] > 0.0f;
-        const float Zm = exp2_hw((G->zeta[iz] - 1.0f) * (pos ? q.Lp : q.Lm));
+        const bool pos = iz < K.nzp;
+        const float Zm = exp2_hw((K.zeta[iz] - 1.0f) * (pos ? q.Lp : q.Lm));
         const float t0 = Zm * u0, t2 = Zm * u2;
-        sc = fmaf(G->K1[iz], t0, sc);
+        sc = fmaf(K.K1[iz], t0, sc);
         if (pos) {
           sep += t0;
           shp += t2;
@@ -852,7 +874,8 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) __attribute__((amdgpu_waves_pe
           op.R = S.R;
           op.r = S.r;
           op.tab = tab;
-          op.G = (bp_grid)md.bpg + (gidx - 1);
+          const bp_grid Gr = (bp_grid)md.bpg + (gidx - 1);
+          op.K.load(Gr);
           op.rc = fn[0].Rc;
           op.g4 = fn[0].kind == 4;
 #pragma unroll
@@ -860,7 +883,7 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) __attribute__((amdgpu_waves_pe
           traverse(op, 0);
           const double G = reduce_scatter<ACH>(op.acc);
           const int slot = lane >> 1;
-          const int fb = op.G->feat[slot];
+          const int fb = Gr->feat[slot];
           if ((lane & 1) == 0 && fb >= 0) {
             const int ft = off + fb;
             const double Gs = md.norm_scale[sp.norm_off + ft] * (G - md.norm_avg[sp.norm_off + ft]);
@@ -1002,10 +1025,11 @@ __global__ __launch_bounds__(64 * ATOM_MAX_WAVES) __attribute__((amdgpu_waves_pe
           op.r = S.r;
           op.tab = tab;
           op.fs = S.fs;
-          op.G = (bp_grid)md.bpg + (bpt->ang_grid[key3] - 1);
+          const bp_grid Gr = (bp_grid)md.bpg + (bpt->ang_grid[key3] - 1);
+          op.K.load(Gr);
 #pragma unroll
           for (int s2 = 0; s2 < ACH; ++s2) {
-            const int fb = op.G->feat[s2];
+            const int fb = Gr->feat[s2];
             op.cs[s2] = fb >= 0 ? coef[off + fb] : 0.0f;
           }
           op.rc = fn[0].Rc;

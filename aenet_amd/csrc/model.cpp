@@ -389,6 +389,10 @@ int PotSet::finalize(std::string &err) {
               if (std::find(lz.begin(), lz.end(), v) == lz.end()) lz.push_back(v);
               if (std::find(et.begin(), et.end(), bpf[k].p3) == et.end()) et.push_back(bpf[k].p3);
             }
+            // lambda = +1 pairs first (the kernel takes the sign from the index)
+            std::stable_sort(lz.begin(), lz.end(), [](const std::pair<float, float> &a, const std::pair<float, float> &b) {
+              return a.first > b.first;
+            });
             if ((int)lz.size() <= BP_GRID_NZ && (int)et.size() <= BP_GRID_NE && (int)(lz.size() * et.size()) == n) {
               BPAngGrid g{};
               g.nz = (int)lz.size();
@@ -402,7 +406,9 @@ int PotSet::finalize(std::string &err) {
                 if (slot >= 0) once = false;
                 slot = k - p0;
               }
+              g.nzp = 0;
               for (int iz = 0; iz < g.nz; ++iz) {
+                if (lz[iz].first > 0.0f) ++g.nzp;
                 g.lam[iz] = lz[iz].first;
                 g.zeta[iz] = lz[iz].second;
                 g.K1[iz] = 0.5f * lz[iz].second * lz[iz].first;
