@@ -154,8 +154,7 @@ struct BPAngGrid {
   int nz, ne;
   int nzp;                 // pairs with lambda = +1, which come first
   float zeta[BP_GRID_NZ];
-  float lam[BP_GRID_NZ];   // +1 or -1
-  float K1[BP_GRID_NZ];    // zeta lambda / 2
+  float K1[BP_GRID_NZ];    // zeta lambda / 2 (lambda = +1 for iz < nzp, else -1)
   float B[BP_GRID_NE];     // -eta log2(e)
   float K2[BP_GRID_NE];    // -2 eta
   int feat[BP_GRID_NZ * BP_GRID_NE];  // place in the block (feature ang_off + feat), -1: none

@@ -409,7 +409,6 @@ int PotSet::finalize(std::string &err) {
               g.nzp = 0;
               for (int iz = 0; iz < g.nz; ++iz) {
                 if (lz[iz].first > 0.0f) ++g.nzp;
-                g.lam[iz] = lz[iz].first;
                 g.zeta[iz] = lz[iz].second;
                 g.K1[iz] = 0.5f * lz[iz].second * lz[iz].first;
               }
