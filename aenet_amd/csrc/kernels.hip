@@ -1,9 +1,9 @@
 // kernels.hip -- gfx950 kernels of the energy/force path (everything but
 // the descriptor/network/force kernels of atom_cheb.hip / atom_bp.hip).
 //
-// Per step:  k_frac -> k_bin_small (or k_frac_bin -> k_scan_cells ->
-//            k_place -> k_cell_sort) -> k_search_cells (compact lists) ->
-//            atom kernel -> k_struct_energy
+// Per step:  k_frac -> k_bin_cells or k_bin_small (or k_frac_bin ->
+//            k_scan_cells -> k_place -> k_cell_sort) -> k_search_cells
+//            (compact lists) -> atom kernel -> k_struct_energy
 // Subsets of atoms (Monte Carlo, autograd): k_search, a wavefront per atom.
 // Once per batch: k_combos (cell/image combinations of each home cell).
 #include <hip/hip_runtime.h>
